@@ -1,0 +1,100 @@
+/*
+ * qs_oracle.h -- FP64 CPU oracle of the Qsolver step (TEST INFRASTRUCTURE).
+ *
+ * This library is the correctness checker for the HIP implementation.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it,
+ * and only as the checker or the timed CPU baseline -- never as the product
+ * path.  The product library (libqsolver.so) does not link or call it.
+ *
+ * It restates, call by call, the reference's single-GPU Qsolver code:
+ *   Basic/Mesh.cuh:6-46                 wave tables and 2/3-rule mask
+ *   FldOp/QFldfuncs.cuh:8-224           transforms and field operations
+ *   InertialFriction/friction/Qsolver.cuh:20-322   physics (canonical copy)
+ *   TimeIntegration/IFE.cuh:15-25, IFRK2.cuh:16-51, RK4.cuh:12-74
+ *   InertialFriction/friction/friction.cu:212-260  per-step call order
+ * with cuFFT replaced by an own mixed-radix FFT (forward e^{-i}, inverse
+ * e^{+i}, both unnormalised, as cuFFT's D2Z / Z2D).
+ *
+ * C2R semantics (cuFFT's treatment of a non-Hermitian half spectrum is not
+ * documented, so the semantics are fixed here and shared with the HIP path):
+ *   f = Re( IDFT_x( G ) ) / (Nx*Ny),  G = IDFT_y(F) column by column, where
+ *   the x-direction C2R discards the imaginary parts of its DC and Nyquist
+ *   bins.  Equivalently: the Hermitian part of column 0 and column Nx/2 is
+ *   taken before a Hermitian C2R.
+ */
+#ifndef QS_ORACLE_H
+#define QS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double x, y; } or_comp;
+
+/* Mesh + Field mirrors (Basic/Mesh.cuh:48-129).  Opaque handles. */
+void*    or_mesh_new(int Nx, int Ny, double Lx, double Ly);
+void     or_mesh_free(void* mesh);
+void     or_tables(void* mesh, double* kx, double* ky, double* k2, double* cutoff);
+void*    or_field_new(void* mesh);
+void     or_field_free(void* field);
+or_comp* or_field_spec(void* field);
+double*  or_field_phys(void* field);
+
+/* Raw 1-D FFT of the oracle (for the brute-force DFT check). sign=-1 fwd. */
+void or_fft1d(int n, int sign, const or_comp* in, or_comp* out);
+
+/* FldOp/QFldfuncs.cuh:45-55 */
+void or_FwdTrans(or_comp* ft, const double* f, void* mesh);
+void or_BwdTrans(double* f, or_comp* ft, void* mesh);
+
+/* InertialFriction/friction/Qsolver.cuh composites, reference signatures
+ * (Field handles in the reference's argument order). */
+void or_single(void* s0, void* r, void* S, double lambda);
+void or_cross(void* cs, void* r1, void* r2, void* aux);
+void or_p11func(void* p11, void* r1, void* S, void* Ra, double lambda);
+void or_p12func(void* p12, void* r1, void* r2, void* S, void* Ra, double lambda,
+                void* aux, void* aux1);
+void or_p21func(void* p21, void* r1, void* r2, void* S, void* Ra, double lambda,
+                void* aux, void* aux1);
+void or_wnonl_func(void* wnonl, void* wcurr, void* u, void* v, void* r1curr,
+                   void* r2curr, void* Ra, void* S, void* p11, void* p12,
+                   void* p21, double Re, double Er, double lambda, void* aux,
+                   void* aux1);
+void or_r1nonl_func(void* r1nonl, void* wcurr, void* u, void* v, void* r1curr,
+                    void* r2curr, void* S, double lambda, void* aux, void* aux1);
+void or_r2nonl_func(void* r2nonl, void* wcurr, void* u, void* v, void* r1curr,
+                    void* r2curr, void* S, double lambda, void* aux, void* aux1);
+void or_curr_func(void* r1curr, void* r2curr, void* wcurr, void* u, void* v,
+                  void* S);
+
+/*
+ * Solver: the whole step with the generalised coefficients.
+ * p[] = {lambda, Re, Er, Rf, alpha, D_Q, a_Q, c_Q, dt, advect_w}
+ * D_Q scales the Laplacian of the Q equation (IF table, single, cross);
+ * a_Q is the Q-linear coefficient; c_Q scales S^2.  D_Q=a_Q=c_Q=1 and
+ * advect_w=1 is exactly the reference (friction/Qsolver.cuh).
+ */
+enum { OR_IFE = 0, OR_IFRK2 = 1, OR_RK4 = 2 };
+void* or_solver_new(void* mesh, const double* p);
+void  or_solver_free(void* s);
+/* Ra: physical activity field (Nx*Ny) or NULL for the uniform alpha. */
+void  or_solver_set_activity(void* s, const double* Ra);
+void  or_solver_set_state(void* s, const or_comp* w, const or_comp* r1, const or_comp* r2);
+void  or_solver_get_state(void* s, or_comp* w, or_comp* r1, or_comp* r2);
+void  or_solver_step(void* s, int method, int nsteps);
+/* One RHS evaluation on given stage spectra (curr_func + w/r1/r2 nonl). The
+ * inputs are Hermitian-symmetrised in column 0 in place (BwdTrans side effect). */
+void  or_solver_rhs(void* s, or_comp* w, or_comp* r1, or_comp* r2,
+                    or_comp* wn, or_comp* r1n, or_comp* r2n);
+/* Integrating-factor tables of B1/B2 (IF = exp(L dt), IFh = exp(L dt/2)). */
+void  or_solver_lin(void* s, double* IFw, double* IFwh, double* IFr1,
+                    double* IFr1h, double* IFr2, double* IFr2h);
+
+/* Threads used by the oracle's OpenMP loops (0 = library default). */
+void  or_set_threads(int n);
+int   or_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
