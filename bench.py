@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 IFRK2 steps/s of the Qsolver at 1024^2 on one MI355X.
+
+Workload (the GPU configuration of the metric): 1024 x 1024 periodic box,
+L = 2 pi, FP64, dt = 5e-4; lambda = 1, Re = 1 (nu = 1), Er = 1, friction 0.1
+(Rf = 0.1), activity alpha = 1, D_Q = K/gamma = 0.05, a_Q = c_Q = 1; band-limited
+random Q (amplitudes exp(-|k|^2/32), uniform random phases, splitmix64 seed 2,
+rms 0.05), omega = 0; 2/3-rule dealiasing.
+
+A step is one fused IFRK2 step (two right-hand sides, six kernels) with the
+state resident in HBM.  W untimed warm-up steps, then K steps timed with HIP
+events between two synchronisations.  Per-kernel durations come from HIP
+events recorded by the library around each launch during the same timed loop.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nature-control_amd"))
+
+HBM_PEAK_GBS = 8000.0
+CONFIG = dict(lam=1.0, Re=1.0, Er=1.0, Rf=0.1, alpha=1.0, D_Q=0.05, a_Q=1.0, c_Q=1.0)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--N", type=int, default=1024)
+    ap.add_argument("--dt", type=float, default=5e-4)
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="IFRK2 steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--unfused", action="store_true",
+                    help="time the per-op (reference call structure) step instead")
+    return ap.parse_args()
+
+
+def cpu_baseline(N, dt, nsteps, q11, q12):
+    """The FP64 CPU oracle on this host's cores, a bounded sample of the same
+    workload (nsteps IFRK2 steps from the same initial state)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle_py as O
+    cores = min(16, os.cpu_count() or 1)
+    O.set_threads(cores)
+    m = O.Mesh(N, N, 2 * np.pi)
+    p = O.params_array(lam=CONFIG["lam"], Re=CONFIG["Re"], Er=CONFIG["Er"], Rf=CONFIG["Rf"],
+                       alpha=CONFIG["alpha"], DQ=CONFIG["D_Q"], aQ=CONFIG["a_Q"], cQ=CONFIG["c_Q"],
+                       dt=dt)
+    s = O.Solver(m, p)
+    s.set_state(np.zeros(m.spec_shape, complex), m.fwd(q11), m.fwd(q12))
+    s.step(1, O.IFRK2)  # warm-up (page-in, thread pool)
+    t0 = time.perf_counter()
+    s.step(nsteps, O.IFRK2)
+    el = time.perf_counter() - t0
+    return {"value": nsteps / el, "unit": "steps/s", "cores": O.get_threads(), "kind": "port",
+            "sample": f"{nsteps} IFRK2 steps at {N}^2 FP64 (same initial state), "
+                      f"{el:.2f} s wall on {O.get_threads()} OpenMP threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the ensemble and batched adjoint shard by run with no exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import qsolver as Q
+    from qsolver import synth
+
+    N, dt = args.N, args.dt
+    H = N * (N // 2 + 1) * 16
+    torch.cuda.init()
+    mesh = Q.Mesh(N, N, 2 * np.pi)
+    q11, q12 = synth.bandlimited_q(N, N, seed=2)
+    f1 = torch.from_numpy(q11).cuda()
+    f2 = torch.from_numpy(q12).cuda()
+    st = [mesh.spec(), mesh.spec(), mesh.spec()]
+    mesh.FwdTrans(st[1], f1)
+    mesh.FwdTrans(st[2], f2)
+    solver = Q.Solver(mesh, Q.params(dt=dt, **CONFIG))
+    solver.set_state(*st)
+    step = solver.step_unfused if args.unfused else solver.step
+
+    step(args.warmup)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if not args.unfused:
+        solver.profile(True)
+    torch.cuda.synchronize()
+    start.record()
+    step(args.steps)
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end)
+    prof = solver.profile_read() if not args.unfused else {}
+    solver.profile(False)
+    finite = not solver.nonfinite()
+
+    steps_per_s = args.steps / (ms / 1e3)
+    # Algorithmic HBM bytes per launch of each kernel class (units of H, the
+    # half-spectrum size N*(N/2+1)*16 B): row pass reads 7 and writes 7 mixed
+    # arrays; each column stage reads 7 mixed arrays + 3 state spectra and
+    # writes 3 state spectra + 7 mixed arrays.
+    alg_H = {"row_rhs": 14, "col_stage1": 20, "col_stage2": 20, "col_prologue": 10}
+    kernels = {}
+    for k, (avg_ms, cnt) in prof.items():
+        if cnt:
+            gbs = alg_H[k] * H / (avg_ms * 1e-3) / 1e9
+            kernels[k] = {"avg_us": round(avg_ms * 1e3, 2), "launches": cnt,
+                          "alg_bytes": alg_H[k] * H, "GBps": round(gbs, 1)}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"]) if kernels else None
+    roofline = None
+    if dom:
+        ach = kernels[dom]["GBps"]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+    step_bytes = 68 * H
+    cpu = None
+    if args.cpu_steps > 0:
+        cpu = cpu_baseline(N, dt, args.cpu_steps, q11, q12)
+    out = {
+        "metric": "IFRK2 steps/s at 1024^2 FP64",
+        "value": round(steps_per_s, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"IFRK2 {N}x{N} FP64 periodic L=2pi, band-limited random Q "
+                               f"(seed 2, rms 0.05), omega=0, dt={dt}",
+                   "N": N, "dt": dt, "path": "unfused" if args.unfused else "fused", **CONFIG},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "step_hbm": {"alg_bytes_per_step": step_bytes, "GBps": round(step_bytes * steps_per_s / 1e9, 1),
+                     "frac_of_peak": round(step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "canonical_101H_frac": round(101 * H * steps_per_s / 1e9 / HBM_PEAK_GBS, 4)},
+        "finite": finite,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

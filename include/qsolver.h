@@ -1,0 +1,214 @@
+/*
+ * qsolver.h -- C-ABI of the MI355X (gfx950) Qsolver library, libqsolver.so.
+ *
+ * Drop-in boundary for the reference's single-GPU Qsolver code
+ * (whpy/nature-control).  Every entry point replaces one reference call; the
+ * comment on each names the reference file:line it stands in for.  Argument
+ * order is the reference's, with the CUDA launch configuration
+ * (<<<grid, block>>>, Nx|Nxh, Ny, BSZ) replaced by the Mesh that carries it.
+ *
+ * Conventions
+ *  - Every pointer to field data is caller-owned DEVICE memory in the
+ *    reference layouts: physical  Qreal  phys[j*Nx + i]   (Nx*Ny values)
+ *                       spectral  Qcomp  spec[j*Nxh + i]  (Nxh*Ny, Nxh=Nx/2+1)
+ *                       wave table Qreal w[j*Nxh + i]     (Nxh*Ny)
+ *  - All calls are asynchronous on the Mesh's HIP stream (qs_mesh_set_stream)
+ *    except the create/destroy calls, qs_solver_get_state's completion and the
+ *    error query.  A Mesh/solver is not thread-safe; distinct objects are.
+ *  - Every function returns a qs_status; nothing terminates the process or
+ *    throws across the boundary (the reference's cudaErr.h:54-72 aborts).
+ *    qs_last_error() returns the message of the calling thread's last failure.
+ *  - In-place aliasing is allowed where the reference aliases (elementwise and
+ *    derivative ops, e.g. friction/Qsolver.cuh:181,238).  FwdTrans/BwdTrans
+ *    need f != ft and preserve their input, except BwdTrans's documented
+ *    column-0 symmetrisation of ft (FldOp/QFldfuncs.cuh:50-55).
+ *  - Supported meshes: Nx, Ny powers of two in [16, 2048].
+ */
+#ifndef QSOLVER_H
+#define QSOLVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double Qreal;                  /* Basic/QActFlowDef.cuh:6 */
+typedef struct { double x, y; } Qcomp; /* Basic/QActFlowDef.cuh:7 (cuDoubleComplex layout) */
+
+typedef enum qs_status {
+  QS_OK = 0,
+  QS_ERR_INVALID_ARG = 1,
+  QS_ERR_UNSUPPORTED_SIZE = 2,
+  QS_ERR_HIP = 3,
+  QS_ERR_NONFINITE = 4,
+  QS_ERR_OOM = 5,
+  QS_ERR_STATE = 6
+} qs_status;
+
+typedef struct qs_mesh_priv qs_mesh_priv;
+
+/* Basic/Mesh.cuh:48-113.  Same field names and order; the dim3 launch shapes
+ * and cufftHandle plans are replaced by the build-private trailing pointer. */
+typedef struct Mesh {
+  int Nx, Nxh, Ny;
+  Qreal Lx, Ly, dx, dy;
+  int specsize, physsize, wavesize;
+  int BSZ;
+  Qcomp* spec;  /* device scratch, Nxh*Ny */
+  Qreal* phys;  /* device scratch, Nx*Ny */
+  Qreal *kx, *ky, *k_squared;
+  Qreal* cutoff;
+  qs_mesh_priv* priv;
+} Mesh;
+
+/* Basic/Mesh.cuh:115-129 */
+typedef struct Field {
+  Qcomp* spec;
+  Qreal* phys;
+  Mesh* mesh;
+} Field;
+
+/* ---- lifecycle --------------------------------------------------------- */
+/* Mesh::Mesh (Basic/Mesh.cuh:61-107): tables (waveinit :6-21, cutoff_func
+ * :23-46), scratch and FFT set-up.  BSZ is kept for the mirror only. */
+int qs_mesh_create(int Nx, int Ny, Qreal Lx, Qreal Ly, int BSZ, Mesh** out);
+/* Mesh::~Mesh (Basic/Mesh.cuh:109-112); also frees what the reference leaks. */
+int qs_mesh_destroy(Mesh* mesh);
+/* No reference counterpart (the reference uses the legacy default stream). */
+int qs_mesh_set_stream(Mesh* mesh, void* hip_stream);
+int qs_mesh_synchronize(const Mesh* mesh);
+/* Field::Field / ~Field (Basic/Mesh.cuh:120-128). */
+int qs_field_create(Mesh* mesh, Field** out);
+int qs_field_destroy(Field* field);
+const char* qs_last_error(void);
+const char* qs_status_string(int status);
+
+/* ---- transforms: FldOp/QFldfuncs.cuh:8-55 ------------------------------- */
+int qs_FwdTrans(Qcomp* ft, const Qreal* f, const Mesh* mesh);     /* :45-48 */
+int qs_BwdTrans(Qreal* f, Qcomp* ft, const Mesh* mesh);           /* :50-55 */
+int qs_coeff(Qreal* f, const Mesh* mesh);                         /* :8-16 */
+int qs_symmetry_func(Qcomp* f_spec, const Mesh* mesh);            /* :18-44 */
+
+/* ---- field algebra: FldOp/QFldfuncs.cuh:63-224 -------------------------- */
+int qs_FldMul(Qreal* pc, const Qreal* pa, const Qreal* pb, Qreal C, const Mesh* mesh);  /* :63-71 */
+int qs_FldAdd(Qreal* pc, Qreal a, const Qreal* pa, Qreal b, const Qreal* pb,
+              const Mesh* mesh);                                                      /* :74-82 */
+int qs_FldAddScalar(Qreal* pc, Qreal a, const Qreal* pa, Qreal b, const Mesh* mesh);   /* :85-93 */
+int qs_FldSet(Qreal* pf, Qreal c, const Mesh* mesh);                                   /* :96-104 */
+int qs_FldCopy(Qreal* pf, const Qreal* c, const Mesh* mesh);                           /* :107-115 */
+int qs_SpecSet(Qcomp* pa, const Qcomp* pb, const Mesh* mesh);                          /* :118-126 */
+int qs_SpecSetConst(Qcomp* pa, Qcomp c, const Mesh* mesh);                             /* :129-137 */
+int qs_SpecAdd(Qcomp* spc, Qreal a, const Qcomp* spa, Qreal b, const Qcomp* spb,
+               const Mesh* mesh);                                                     /* :141-149 */
+int qs_SpecMul(Qcomp* spc, const Qcomp* spa, Qreal C, const Mesh* mesh);               /* :151-159 */
+int qs_xDeriv(Qcomp* fx, const Qcomp* f, const Mesh* mesh);                            /* :165-176 */
+int qs_yDeriv(Qcomp* fy, const Qcomp* f, const Mesh* mesh);                            /* :178-189 */
+int qs_Laplacian(Qcomp* fL, const Qcomp* f, const Mesh* mesh);                         /* :191-202 */
+int qs_vel_func(const Qcomp* w_c, Qcomp* u_c, Qcomp* v_c, const Mesh* mesh);           /* :204-224 */
+
+/* ---- physics: InertialFriction/friction/Qsolver.cuh --------------------- */
+int qs_wlin_func(Qreal* IFw, Qreal* IFwh, Qreal Re, Qreal Rf, Qreal dt, const Mesh* mesh); /* :20-30 */
+int qs_r1lin_func(Qreal* IFr1, Qreal* IFr1h, Qreal dt, const Mesh* mesh);                 /* :32-42 */
+int qs_r2lin_func(Qreal* IFr2, Qreal* IFr2h, Qreal dt, const Mesh* mesh);                 /* :44-54 */
+int qs_S_func(Qreal* S, const Qreal* r1, const Qreal* r2, const Mesh* mesh);              /* :63-71 */
+int qs_Rar(Qreal* rar, const Qreal* r, const Qreal* Ra, const Mesh* mesh);                /* :75-83 */
+int qs_single1(Qreal* s1, const Qreal* r, const Qreal* S, Qreal lambda, const Mesh* mesh); /* :86-94 */
+int qs_S2r(Qreal* rs, const Qreal* r, const Qreal* S, const Mesh* mesh);                  /* :206-215 */
+int qs_PiDeriv(Qcomp* dP, const Qcomp* p12, const Qcomp* p11, const Qcomp* p21, Qreal Er,
+               Qreal Re, const Mesh* mesh);                                               /* :154-164 */
+int qs_dealias(Qcomp* ft, const Mesh* mesh);                                              /* :314-322 */
+int qs_single(Field* s0, Field* r, Field* S, Qreal lambda);                                /* :96-105 */
+int qs_cross(Field* cs, Field* r1, Field* r2, Field* aux);                                 /* :107-122 */
+int qs_p11func(Field* p11, Field* r1, Field* S, Field* Ra, Qreal lambda);                  /* :124-131 */
+int qs_p12func(Field* p12, Field* r1, Field* r2, Field* S, Field* Ra, Qreal lambda,
+               Field* aux, Field* aux1);                                                  /* :133-141 */
+int qs_p21func(Field* p21, Field* r1, Field* r2, Field* S, Field* Ra, Qreal lambda,
+               Field* aux, Field* aux1);                                                  /* :143-151 */
+int qs_wnonl_func(Field* wnonl, Field* wcurr, Field* u, Field* v, Field* r1curr,
+                  Field* r2curr, Field* Ra, Field* S, Field* p11, Field* p12, Field* p21,
+                  Qreal Re, Qreal Er, Qreal lambda, Field* aux, Field* aux1);             /* :166-201 */
+int qs_r1nonl_func(Field* r1nonl, Field* wcurr, Field* u, Field* v, Field* r1curr,
+                   Field* r2curr, Field* S, Qreal lambda, Field* aux, Field* aux1);       /* :217-255 */
+int qs_r2nonl_func(Field* r2nonl, Field* wcurr, Field* u, Field* v, Field* r1curr,
+                   Field* r2curr, Field* S, Qreal lambda, Field* aux, Field* aux1);       /* :257-300 */
+int qs_curr_func(Field* r1curr, Field* r2curr, Field* wcurr, Field* u, Field* v,
+                 Field* S);                                                               /* :302-312 */
+
+/* ---- time integration: TimeIntegration/{IFE,IFRK2,RK4}.cuh ------------- */
+int qs_ife_func1D(const Qcomp* old_spec, Qcomp* new_spec, const Qcomp* nonl_spec,
+                  const Qreal* IF, const Qreal* IFh, Qreal dt, const Mesh* mesh);   /* IFE.cuh:15-25 */
+int qs_ifrk2_func0D(const Qcomp* old_spec, Qcomp* curr_spec, Qcomp* new_spec,
+                    const Qreal* IF, const Qreal* IFh, Qreal dt, const Mesh* mesh); /* IFRK2.cuh:16-26 */
+int qs_ifrk2_func1D(const Qcomp* old_spec, Qcomp* curr_spec, Qcomp* new_spec,
+                    const Qcomp* nonl_spec, const Qreal* IF, const Qreal* IFh, Qreal dt,
+                    const Mesh* mesh);                                              /* IFRK2.cuh:28-39 */
+int qs_ifrk2_func2D(const Qcomp* old_spec, Qcomp* curr_spec, Qcomp* new_spec,
+                    const Qcomp* nonl_spec, const Qreal* IF, const Qreal* IFh, Qreal dt,
+                    const Mesh* mesh);                                              /* IFRK2.cuh:41-51 */
+int qs_rk4_func0D(const Qcomp* old_spec, Qcomp* curr_spec, Qcomp* new_spec,
+                  const Qreal* IF, const Qreal* IFh, Qreal dt, const Mesh* mesh);   /* RK4.cuh:12-22 */
+int qs_rk4_func1D(const Qcomp* old_spec, Qcomp* curr_spec, Qcomp* new_spec,
+                  const Qcomp* nonl_spec, const Qreal* IF, const Qreal* IFh, Qreal dt,
+                  const Mesh* mesh);                                                /* RK4.cuh:24-35 */
+int qs_rk4_func2D(const Qcomp* old_spec, Qcomp* curr_spec, Qcomp* new_spec,
+                  const Qcomp* nonl_spec, const Qreal* IF, const Qreal* IFh, Qreal dt,
+                  const Mesh* mesh);                                                /* RK4.cuh:37-48 */
+int qs_rk4_func3D(const Qcomp* old_spec, Qcomp* curr_spec, Qcomp* new_spec,
+                  const Qcomp* nonl_spec, const Qreal* IF, const Qreal* IFh, Qreal dt,
+                  const Mesh* mesh);                                                /* RK4.cuh:50-61 */
+int qs_rk4_func4D(const Qcomp* old_spec, Qcomp* curr_spec, Qcomp* new_spec,
+                  const Qcomp* nonl_spec, const Qreal* IF, const Qreal* IFh, Qreal dt,
+                  const Mesh* mesh);                                                /* RK4.cuh:63-74 */
+
+/* ---- fused solver ------------------------------------------------------- */
+/* Parameters of InertialFriction/friction/friction.cu:100-105 plus the
+ * Q-equation coefficients D_Q (Laplacian), a_Q (linear) and c_Q (S^2 term);
+ * D_Q = a_Q = c_Q = 1 and advect_w = 1 is exactly the reference physics
+ * (friction/Qsolver.cuh:38,50,213; advect_w = 0 is beginning/Qsolver.cuh:192). */
+typedef struct qs_params {
+  Qreal lambda, Re, Er, Rf, alpha;
+  Qreal D_Q, a_Q, c_Q;
+  Qreal dt;
+  int advect_w;
+  int reserved;
+} qs_params;
+
+typedef struct qs_solver qs_solver;
+
+/* Owns the state (old/new/curr spectra of omega, r1, r2), the IF tables and
+ * the workspaces of the fused kernels.  Replaces the Field set-up and IF
+ * tables of friction.cu:150-210. */
+int qs_solver_create(Mesh* mesh, const qs_params* params, qs_solver** out);
+int qs_solver_destroy(qs_solver* s);
+/* friction.cu:191-192 (Ra field).  Ra: device Nx*Ny, or NULL = uniform alpha. */
+int qs_solver_set_activity(qs_solver* s, const Qreal* Ra);
+/* Device spectra in the reference layout (copied in / out). */
+int qs_solver_set_state(qs_solver* s, const Qcomp* w, const Qcomp* r1, const Qcomp* r2);
+int qs_solver_get_state(qs_solver* s, Qcomp* w, Qcomp* r1, Qcomp* r2);
+/* nsteps IFRK2 steps (friction.cu:212-260 with TimeIntegration/IFRK2.cuh:16-51,
+ * IFh := exp(L dt)), fused into six kernels per step. */
+int qs_step_ifrk2(qs_solver* s, int nsteps);
+/* Same step built from the per-op entry points above (reference call
+ * granularity); for parity and for timing the unfused structure. */
+int qs_step_ifrk2_unfused(qs_solver* s, int nsteps);
+/* 1 if a non-finite value reached the state since the last query. */
+int qs_solver_nonfinite(qs_solver* s, int* flag);
+/* No reference counterpart (the reference has no timers).  While enabled,
+ * every kernel of qs_step_ifrk2 is bracketed by HIP events on the Mesh's
+ * stream; qs_solver_profile_read synchronises and returns the average
+ * duration in ms and the launch count of each kernel class
+ * [0] column prologue, [1] row pass, [2] column stage 1, [3] column stage 2,
+ * then forgets the recorded events. */
+int qs_solver_profile(qs_solver* s, int enable);
+int qs_solver_profile_read(qs_solver* s, double* avg_ms, int* counts);
+
+/* ---- host helpers (no GPU) ---------------------------------------------- */
+/* splitmix64 uniform [0,1) stream: u_k = (x_k >> 11) * 2^-53. */
+int qs_synth_uniform(uint64_t seed, Qreal lo, Qreal hi, size_t n, Qreal* host_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSOLVER_H */

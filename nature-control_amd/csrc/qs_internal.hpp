@@ -1,0 +1,215 @@
+// qs_internal.hpp -- shared host/device definitions of libqsolver (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "qsolver.h"
+
+// ---------------------------------------------------------------------------
+// Error handling: thread-local message + status codes (never abort).
+// ---------------------------------------------------------------------------
+namespace qs {
+
+void set_error(const char* fmt, ...);
+int fail(int status, const char* fmt, ...);
+
+#define QS_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return ::qs::fail(QS_ERR_HIP, "%s:%d %s -> %s", __FILE__, __LINE__, #call,       \
+                        hipGetErrorString(e_));                                        \
+  } while (0)
+
+#define QS_CHECK_LAUNCH() QS_HIP(hipGetLastError())
+
+#define QS_TRY(expr)              \
+  do {                            \
+    int s_ = (expr);              \
+    if (s_ != QS_OK) return s_;   \
+  } while (0)
+
+inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+inline int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
+
+}  // namespace qs
+
+// Build-private part of the Mesh.
+struct qs_mesh_priv {
+  hipStream_t stream = nullptr;
+  double2* twx = nullptr;   // W_Nx^m = exp(-2 pi i m/Nx), m < Nx
+  double2* twy = nullptr;   // W_Ny^m
+  double* kx1 = nullptr;    // kx(i), i < Nxh   (1-D, separable tables)
+  double* ky1 = nullptr;    // ky(j), j < Ny
+};
+
+namespace qs {
+
+inline hipStream_t stream_of(const Mesh* m) { return m->priv->stream; }
+int check_mesh(const Mesh* m);
+
+// Per-op 2-D transforms in the reference layouts (qs_fft.hip).
+int fft_r2c_2d(const Mesh* m, const double* f, double2* mix, double2* ft);
+int fft_c2r_2d(const Mesh* m, double2* ft, double2* mix, double* f, bool symmetrise_input);
+
+// Coefficient-aware elementwise forms (qs_fldops.hip).
+int qlin_c(double* IF, double* IFh, double dt, double DQ, double aQ, const Mesh* mesh);
+int single1_c(double* s1, const double* r, const double* S, double lambda, double cQ, double aQ,
+              const Mesh* mesh);
+int S2r_c(double* rs, const double* r, const double* S, double cQ, const Mesh* mesh);
+
+// Generalised Q-equation coefficients; all 1 (and advect_w) = reference.
+struct Coef {
+  double DQ = 1.0, aQ = 1.0, cQ = 1.0;
+  bool advect_w = true;
+};
+
+// Coefficient-aware composites of friction/Qsolver.cuh (qs_physics.cpp).
+int single_c(Field* s0, Field* r, Field* S, Qreal lambda, const Coef& c);
+int cross_c(Field* cs, Field* r1, Field* r2, Field* aux, const Coef& c);
+int p11func_c(Field* p11, Field* r1, Field* S, Field* Ra, Qreal lambda, const Coef& c);
+int p12func_c(Field* p12, Field* r1, Field* r2, Field* S, Field* Ra, Qreal lambda, Field* aux,
+              Field* aux1, const Coef& c);
+int p21func_c(Field* p21, Field* r1, Field* r2, Field* S, Field* Ra, Qreal lambda, Field* aux,
+              Field* aux1, const Coef& c);
+int wnonl_func_c(Field* wnonl, Field* wcurr, Field* u, Field* v, Field* r1curr, Field* r2curr,
+                 Field* Ra, Field* S, Field* p11, Field* p12, Field* p21, Qreal Re, Qreal Er,
+                 Qreal lambda, Field* aux, Field* aux1, const Coef& c);
+int r1nonl_func_c(Field* r1nonl, Field* wcurr, Field* u, Field* v, Field* r1curr, Field* r2curr,
+                  Field* S, Qreal lambda, Field* aux, Field* aux1, const Coef& c);
+int r2nonl_func_c(Field* r2nonl, Field* wcurr, Field* u, Field* v, Field* r1curr, Field* r2curr,
+                  Field* S, Qreal lambda, Field* aux, Field* aux1, const Coef& c);
+
+// Host twiddle table W_n^m = exp(-2 pi i m / n), accurate to ~0.5 ulp.
+void host_twiddles(int n, double2* out);
+
+// ---------------------------------------------------------------------------
+// Device complex helpers (double2 = {re, im}).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// multiply by i (SIGN>0) or -i (SIGN<0)
+template <int SIGN>
+__device__ __forceinline__ double2 mul_pmi(double2 a) {
+  return SIGN > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------------------
+// Small DFTs in registers.  SIGN = -1 forward (e^{-i}), +1 inverse.
+// ---------------------------------------------------------------------------
+template <int SIGN>
+__device__ __forceinline__ void dft2(double2& a, double2& b) {
+  double2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft4(double2* v) {
+  double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+  double2 s13 = cadd(v[1], v[3]), d13 = mul_pmi<SIGN>(csub(v[1], v[3]));
+  v[0] = cadd(s02, s13);
+  v[2] = csub(s02, s13);
+  v[1] = cadd(d02, d13);
+  v[3] = csub(d02, d13);
+}
+
+// radix-8 as 2 x 4 with the W8 twiddles applied between the stages.
+template <int SIGN>
+__device__ __forceinline__ void dft8(double2* v) {
+  constexpr double H = 0.70710678118654752440;  // cos(pi/4)
+  double2 a[4] = {v[0], v[2], v[4], v[6]};
+  double2 b[4] = {v[1], v[3], v[5], v[7]};
+  dft4<SIGN>(a);
+  dft4<SIGN>(b);
+  // b[k] *= W8^k (SIGN)
+  b[1] = SIGN < 0 ? make_double2(H * (b[1].x + b[1].y), H * (b[1].y - b[1].x))
+                  : make_double2(H * (b[1].x - b[1].y), H * (b[1].y + b[1].x));
+  b[2] = mul_pmi<SIGN>(b[2]);
+  b[3] = SIGN < 0 ? make_double2(H * (b[3].y - b[3].x), -H * (b[3].x + b[3].y))
+                  : make_double2(-H * (b[3].x + b[3].y), H * (b[3].x - b[3].y));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = cadd(a[k], b[k]);
+    v[k + 4] = csub(a[k], b[k]);
+  }
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_small(double2* v) {
+  if constexpr (R == 2) dft2<SIGN>(v[0], v[1]);
+  else if constexpr (R == 4) dft4<SIGN>(v);
+  else if constexpr (R == 8) dft8<SIGN>(v);
+}
+
+// ---------------------------------------------------------------------------
+// In-LDS Stockham FFT of NV vectors of length N (element e of vector v at
+// s[v*LD + e]), executed by NT threads.  Radix RMAX passes while they fit,
+// then one smaller pass.  Twiddles W_N^m from `tw` (global, L1/L2 resident).
+// Output in natural order, in place.  Ends with a barrier.
+// ---------------------------------------------------------------------------
+template <int N, int R, int P, int SIGN, int NV, int LD, int NT>
+__device__ __forceinline__ void stockham_pass(double2* s, const double2* __restrict__ tw, int tid) {
+  constexpr int Q = N / R;             // butterflies per vector
+  constexpr int NB = NV * Q;           // butterflies in total
+  constexpr int PER = (NB + NT - 1) / NT;
+  constexpr int STEP = N / (P * R);
+  double2 v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) {
+      const int vec = b / Q, i = b % Q;
+      const int k = i & (P - 1);
+      const double2* src = s + vec * LD + i;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = src[r * Q];
+      if (P > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          double2 w = tw[r * k * STEP];
+          if (SIGN > 0) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      dft_small<R, SIGN>(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) {
+      const int vec = b / Q, i = b % Q;
+      const int k = i & (P - 1);
+      double2* dst = s + vec * LD + (i - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * P] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int RMAX, int P, int SIGN, int NV, int LD, int NT>
+__device__ __forceinline__ void stockham_from(double2* s, const double2* __restrict__ tw, int tid) {
+  if constexpr (P < N) {
+    constexpr int R = (N / P >= RMAX) ? RMAX : (N / P);
+    stockham_pass<N, R, P, SIGN, NV, LD, NT>(s, tw, tid);
+    stockham_from<N, RMAX, P * R, SIGN, NV, LD, NT>(s, tw, tid);
+  }
+}
+
+template <int N, int RMAX, int SIGN, int NV, int LD, int NT>
+__device__ __forceinline__ void fft_lds(double2* s, const double2* __restrict__ tw, int tid) {
+  stockham_from<N, RMAX, 1, SIGN, NV, LD, NT>(s, tw, tid);
+}
+
+}  // namespace qs
