@@ -50,7 +50,7 @@ __global__ __launch_bounds__(RowCfg<N>::NT) void k_row_r2c(const double* __restr
     s[vec * N + x] = make_double2(a, b);
   }
   __syncthreads();
-  fft_lds<N, kRadix, -1, C::NV, N, C::NT>(s, tw, tid);
+  fft_lds<N, kRadix, -1, C::NV, N, C::NT>(s, TwFull{tw}, tid);
   for (int e = tid; e < C::NV * H; e += C::NT) {
     const int vec = e / H, k = e % H;
     const int y0 = 2 * (pair0 + vec), y1 = y0 + 1;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(RowCfg<N>::NT) void k_row_c2r(const double2* __rest
     }
   }
   __syncthreads();
-  fft_lds<N, kRadix, +1, C::NV, N, C::NT>(s, tw, tid);
+  fft_lds<N, kRadix, +1, C::NV, N, C::NT>(s, TwFull{tw}, tid);
   for (int e = tid; e < C::NV * N; e += C::NT) {
     const int vec = e / N, x = e % N;
     const int y0 = 2 * (pair0 + vec), y1 = y0 + 1;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(ColCfg<NY>::NT) void k_col_fft(const double2* in, d
       sym[(size_t)(NY - j) * Nxh] = cconj(m);
     }
   }
-  fft_lds<NY, kRadix, SIGN, C::TC, C::LD, C::NT>(s, tw, tid);
+  fft_lds<NY, kRadix, SIGN, C::TC, C::LD, C::NT>(s, TwFull{tw}, tid);
   for (int e = tid; e < C::TC * NY; e += C::NT) {
     const int c = e % C::TC, j = e / C::TC, i = i0 + c;
     if (i < Nxh) out[(size_t)j * Nxh + i] = s[c * C::LD + j];

@@ -1,0 +1,39 @@
+// qs_fused.hpp -- descriptors and launchers of the fused IFRK2 kernels.
+#pragma once
+
+#include "qs_internal.hpp"
+
+namespace qs {
+
+enum { MODE_PRO = 0, MODE_S1 = 1, MODE_S2 = 2 };
+
+// Everything a fused job needs about one run (device-resident array).
+struct RunDesc {
+  int Nx, Ny, Nxh, pitch;         // pitch: row stride (complex) of state and mixed arrays
+  double2* mix[7];                // mixed arrays [Ny][pitch]
+  double2* old_[3];               // state w, r1, r2 [Ny][pitch]
+  double2* new_[3];               // stage-1 partial sums
+  const double* Ra;               // activity field [Ny][Nx] or null (uniform alpha)
+  double* w00;                    // W(0,0) of the current stage
+  int* flag;                      // non-finite flag
+  const double* kx1;              // kx(i), i < Nxh
+  const double* ky1;              // ky(j), j < Ny
+  double alpha, lambda, DQ, aQ, cQ, s_ErRe, scale, dt, Re, Rf;
+  int advect_w;
+};
+
+// Jobs of one run per launch (row: 1024/Nx rows each; column: 3 fields x tiles).
+int row_jobs(const RunDesc& a);
+int col_jobs(const RunDesc& a);
+
+// work[k] = (run index, job index).  grid <= nwork; blocks loop over the list.
+// nfix: the common mesh size of all listed runs (Nx = Ny = nfix), or 0 for a
+// list of runs of different sizes (per-run dispatch inside the kernel).
+int launch_fused_row(int nfix, const RunDesc* runs, const int2* work, int nwork, int grid,
+                     const double2* tw1024, hipStream_t st);
+int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, int nwork, int grid,
+                     const double2* tw1024, hipStream_t st);
+
+inline int pitch_of(int Nxh) { return (Nxh + 7) & ~7; }
+
+}  // namespace qs

@@ -44,6 +44,7 @@ struct qs_mesh_priv {
   double2* twy = nullptr;   // W_Ny^m
   double* kx1 = nullptr;    // kx(i), i < Nxh   (1-D, separable tables)
   double* ky1 = nullptr;    // ky(j), j < Ny
+  double2* tw1024 = nullptr; // W_1024^m, m < 1024 (fused kernels use the first 256)
 };
 
 namespace qs {
@@ -151,13 +152,44 @@ __device__ __forceinline__ void dft_small(double2* v) {
 }
 
 // ---------------------------------------------------------------------------
+// Twiddle accessors: W_N^m = exp(-2 pi i m / N), 0 <= m < N.
+//   TwFull:    table of all N entries (global or LDS).
+//   TwQuarter: table of the first N/4 entries; W^m = (-i)^q W^r with
+//              q = m / (N/4), r = m % (N/4) (exact quadrant rotation).
+// ---------------------------------------------------------------------------
+struct TwFull {
+  const double2* p;
+  __device__ __forceinline__ double2 operator()(int m) const { return p[m]; }
+};
+
+template <int N>
+struct TwQuarter {
+  const double2* p;
+  __device__ __forceinline__ double2 operator()(int m) const {
+    constexpr int Q4 = N / 4;
+    const int q = m / Q4, r = m % Q4;
+    const double2 w = p[r];
+    if (q == 0) return w;
+    if (q == 1) return make_double2(w.y, -w.x);
+    if (q == 2) return make_double2(-w.x, -w.y);
+    return make_double2(-w.y, w.x);
+  }
+};
+
+// Cooperative copy of the first n twiddles into LDS (caller synchronises).
+__device__ __forceinline__ void load_twiddles(double2* dst, const double2* __restrict__ src, int n,
+                                              int tid, int nt) {
+  for (int m = tid; m < n; m += nt) dst[m] = src[m];
+}
+
+// ---------------------------------------------------------------------------
 // In-LDS Stockham FFT of NV vectors of length N (element e of vector v at
 // s[v*LD + e]), executed by NT threads.  Radix RMAX passes while they fit,
-// then one smaller pass.  Twiddles W_N^m from `tw` (global, L1/L2 resident).
-// Output in natural order, in place.  Ends with a barrier.
+// then one smaller pass.  Output in natural order, in place.  Ends with a
+// barrier.
 // ---------------------------------------------------------------------------
-template <int N, int R, int P, int SIGN, int NV, int LD, int NT>
-__device__ __forceinline__ void stockham_pass(double2* s, const double2* __restrict__ tw, int tid) {
+template <int N, int R, int P, int SIGN, int NV, int LD, int NT, class TW>
+__device__ __forceinline__ void stockham_pass(double2* s, const TW& tw, int tid) {
   constexpr int Q = N / R;             // butterflies per vector
   constexpr int NB = NV * Q;           // butterflies in total
   constexpr int PER = (NB + NT - 1) / NT;
@@ -175,7 +207,7 @@ __device__ __forceinline__ void stockham_pass(double2* s, const double2* __restr
       if (P > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          double2 w = tw[r * k * STEP];
+          double2 w = tw(r * k * STEP);
           if (SIGN > 0) w.y = -w.y;
           v[q][r] = cmul(v[q][r], w);
         }
@@ -198,8 +230,8 @@ __device__ __forceinline__ void stockham_pass(double2* s, const double2* __restr
   __syncthreads();
 }
 
-template <int N, int RMAX, int P, int SIGN, int NV, int LD, int NT>
-__device__ __forceinline__ void stockham_from(double2* s, const double2* __restrict__ tw, int tid) {
+template <int N, int RMAX, int P, int SIGN, int NV, int LD, int NT, class TW>
+__device__ __forceinline__ void stockham_from(double2* s, const TW& tw, int tid) {
   if constexpr (P < N) {
     constexpr int R = (N / P >= RMAX) ? RMAX : (N / P);
     stockham_pass<N, R, P, SIGN, NV, LD, NT>(s, tw, tid);
@@ -207,9 +239,142 @@ __device__ __forceinline__ void stockham_from(double2* s, const double2* __restr
   }
 }
 
-template <int N, int RMAX, int SIGN, int NV, int LD, int NT>
-__device__ __forceinline__ void fft_lds(double2* s, const double2* __restrict__ tw, int tid) {
+template <int N, int RMAX, int SIGN, int NV, int LD, int NT, class TW>
+__device__ __forceinline__ void fft_lds(double2* s, const TW& tw, int tid) {
   stockham_from<N, RMAX, 1, SIGN, NV, LD, NT>(s, tw, tid);
+}
+
+// WG id -> logical id such that the WGs one XCD runs (ids congruent mod 8,
+// dispatched round-robin over the 8 XCDs) get a contiguous logical range, so
+// neighbouring column tiles share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+  const int per = G / 8, rem = G % 8;
+  const int x = b % 8, slot = b / 8;
+  return x * per + (x < rem ? x : rem) + slot;
+}
+
+}  // namespace qs
+
+namespace qs {
+
+// ---------------------------------------------------------------------------
+// Radix-16 DFT in registers: 4 x 4 with the W16 twiddles between the stages.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 w16(int m, int sign) {
+  constexpr double C1 = 0.92387953251128675613;  // cos(pi/8)
+  constexpr double S1 = 0.38268343236508977173;  // sin(pi/8)
+  constexpr double H = 0.70710678118654752440;   // cos(pi/4)
+  double c = 1.0, s = 0.0;                        // W16^m = c - i s (forward)
+  switch (m) {
+    case 1: c = C1; s = S1; break;
+    case 2: c = H; s = H; break;
+    case 3: c = S1; s = C1; break;
+    case 4: c = 0.0; s = 1.0; break;
+    case 6: c = -H; s = H; break;
+    case 9: c = -C1; s = -S1; break;
+    default: break;
+  }
+  return make_double2(c, sign < 0 ? -s : s);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft16(double2* v) {
+  double2 a[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    double2 t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+    dft4<SIGN>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = (n2 * k1 == 0) ? t[k1] : cmul(t[k1], w16(n2 * k1, SIGN));
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    double2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+    dft4<SIGN>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+  }
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_any(double2* v) {
+  if constexpr (R == 16) dft16<SIGN>(v);
+  else dft_small<R, SIGN>(v);
+}
+
+// Orders LDS traffic of one wavefront: prior LDS writes of the wave are
+// visible to its later LDS reads and no access moves across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS index with one pad slot per 16 elements (breaks the stride-16 pattern
+// of the first Stockham write across the 64 lanes).
+__device__ __forceinline__ int swz(int e) { return e + (e >> 4); }
+
+// ---------------------------------------------------------------------------
+// Per-wavefront FFT of CW = 1024/N independent length-N vectors (16 <= N <=
+// 1024).  Lane (lp, g) with lp < LPC = N/16 holds elements lp + LPC*m,
+// m < 16, of vector g in v[]; on return it holds the transform at the same
+// positions.  Radix-16 first pass in registers, then radix-min(16, N/P)
+// Stockham passes through `buf` (this wave's LDS region, CW*(N + N/16)
+// entries, vector g at offset g*(N + N/16)).  Only wave-level ordering is
+// needed: no workgroup barrier.
+// ---------------------------------------------------------------------------
+template <int N, int P, int SIGN, class TW>
+__device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, int lp, const TW& tw) {
+  constexpr int LPC = N / 16;
+  constexpr int R = (N / P >= 16) ? 16 : N / P;
+  constexpr int Q = N / R;
+  constexpr int BPL = Q / LPC;          // butterflies per lane
+  constexpr bool LAST = (P * R == N);
+  constexpr int STEP = N / (P * R);
+  double2 t[BPL][R];
+#pragma unroll
+  for (int q = 0; q < BPL; ++q) {
+    const int b = lp + LPC * q;
+    const int k = b & (P - 1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) t[q][r] = buf[swz(b + r * Q)];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      double2 w = tw(r * k * STEP);
+      if (SIGN > 0) w.y = -w.y;
+      t[q][r] = cmul(t[q][r], w);
+    }
+    dft_any<R, SIGN>(t[q]);
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int q = 0; q < BPL; ++q)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q + r * BPL] = t[q][r];
+  } else {
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < BPL; ++q) {
+      const int b = lp + LPC * q;
+      const int k = b & (P - 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[swz((b - k) * R + k + r * P)] = t[q][r];
+    }
+    wave_sync();
+    wave_fft_pass<N, P * R, SIGN>(v, buf, lp, tw);
+  }
+}
+
+template <int N, int SIGN, class TW>
+__device__ __forceinline__ void wave_fft(double2 (&v)[16], double2* buf, int lp, const TW& tw) {
+  dft16<SIGN>(v);                          // pass 1 (P = 1): butterfly lp
+  if constexpr (N > 16) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) buf[swz(lp * 16 + r)] = v[r];
+    wave_sync();
+    wave_fft_pass<N, 16, SIGN>(v, buf, lp, tw);
+    wave_sync();                           // buf may be rewritten by the caller
+  }
 }
 
 }  // namespace qs
