@@ -203,6 +203,15 @@ int qs_solver_nonfinite(qs_solver* s, int* flag);
  * then forgets the recorded events. */
 int qs_solver_profile(qs_solver* s, int enable);
 int qs_solver_profile_read(qs_solver* s, double* avg_ms, int* counts);
+/* Test hooks (no reference counterpart): run one kernel of the fused step
+ * (0 column prologue, 1 row pass, 2 column stage 1, 3 column stage 2) and
+ * copy an internal array out in the reference layout (0..6 mixed arrays,
+ * 7..9 state w/r1/r2, 10..12 stage-1 partial sums). */
+int qs_solver_debug_phase(qs_solver* s, int phase);
+int qs_solver_debug_array(qs_solver* s, int which, Qcomp* out);
+/* Test hook: the fused kernels' per-wavefront FFT (length n, 16 <= n <= 1024,
+ * sign -1 forward / +1 inverse, unnormalised) on 1024/n contiguous vectors. */
+int qs_debug_wave_fft(const Mesh* mesh, int n, int sign, const Qcomp* in, Qcomp* out);
 
 /* ---- host helpers (no GPU) ---------------------------------------------- */
 /* splitmix64 uniform [0,1) stream: u_k = (x_k >> 11) * 2^-53. */

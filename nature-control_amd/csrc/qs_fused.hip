@@ -447,3 +447,45 @@ int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, 
 }
 
 }  // namespace qs
+
+// ---------------------------------------------------------------------------
+// Test hook: per-wavefront FFT of CW = 1024/N vectors (one wave, one block).
+// in/out: CW vectors of length N, contiguous.
+// ---------------------------------------------------------------------------
+namespace qs {
+namespace {
+template <int N, int SIGN>
+__global__ __launch_bounds__(64) void k_wave_fft_test(const double2* in, double2* out, const double2* tw1024) {
+  __shared__ double2 sm[kBuf];
+  __shared__ double2 twq[kTwq];
+  stage_twiddles(twq, tw1024, threadIdx.x, 64);
+  __syncthreads();
+  constexpr int CW = 1024 / N, LPC = N / 16, VS = N + N / 16;
+  const int lane = threadIdx.x, lp = lane / CW, g = lane % CW;
+  double2 v[16];
+  for (int m = 0; m < 16; ++m) v[m] = in[g * N + lp + LPC * m];
+  wave_fft<N, SIGN>(v, sm + g * VS, lp, TwN<N>{twq});
+  for (int m = 0; m < 16; ++m) out[g * N + lp + LPC * m] = v[m];
+}
+template <int N>
+int launch_wave_fft_test(int sign, const double2* in, double2* out, const double2* tw, hipStream_t st) {
+  if (sign < 0) hipLaunchKernelGGL((k_wave_fft_test<N, -1>), dim3(1), dim3(64), 0, st, in, out, tw);
+  else hipLaunchKernelGGL((k_wave_fft_test<N, +1>), dim3(1), dim3(64), 0, st, in, out, tw);
+  QS_CHECK_LAUNCH();
+  return QS_OK;
+}
+}  // namespace
+
+int wave_fft_test(int n, int sign, const double2* in, double2* out, const double2* tw1024, hipStream_t st) {
+  switch (n) {
+    case 16: return launch_wave_fft_test<16>(sign, in, out, tw1024, st);
+    case 32: return launch_wave_fft_test<32>(sign, in, out, tw1024, st);
+    case 64: return launch_wave_fft_test<64>(sign, in, out, tw1024, st);
+    case 128: return launch_wave_fft_test<128>(sign, in, out, tw1024, st);
+    case 256: return launch_wave_fft_test<256>(sign, in, out, tw1024, st);
+    case 512: return launch_wave_fft_test<512>(sign, in, out, tw1024, st);
+    case 1024: return launch_wave_fft_test<1024>(sign, in, out, tw1024, st);
+    default: return fail(QS_ERR_UNSUPPORTED_SIZE, "wave fft %d", n);
+  }
+}
+}  // namespace qs

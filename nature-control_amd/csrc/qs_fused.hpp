@@ -34,6 +34,8 @@ int launch_fused_row(int nfix, const RunDesc* runs, const int2* work, int nwork,
 int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, int nwork, int grid,
                      const double2* tw1024, hipStream_t st);
 
+int wave_fft_test(int n, int sign, const double2* in, double2* out, const double2* tw1024, hipStream_t st);
+
 inline int pitch_of(int Nxh) { return (Nxh + 7) & ~7; }
 
 }  // namespace qs

@@ -167,12 +167,16 @@ struct TwQuarter {
   const double2* p;
   __device__ __forceinline__ double2 operator()(int m) const {
     constexpr int Q4 = N / 4;
-    const int q = m / Q4, r = m % Q4;
+    const int q = (m / Q4) & 3, r = m % Q4;
     const double2 w = p[r];
-    if (q == 0) return w;
-    if (q == 1) return make_double2(w.y, -w.x);
-    if (q == 2) return make_double2(-w.x, -w.y);
-    return make_double2(-w.y, w.x);
+    // (-i)^q w without branches: q odd swaps the parts; signs
+    // q=0 (+,+), q=1 (+,-), q=2 (-,-), q=3 (-,+).
+    const bool odd = q & 1;
+    const double a = odd ? w.y : w.x;
+    const double b = odd ? w.x : w.y;
+    const double sa = (q & 2) ? -1.0 : 1.0;
+    const double sb = ((q ^ (q >> 1)) & 1) ? -1.0 : 1.0;
+    return make_double2(sa * a, sb * b);
   }
 };
 
@@ -302,12 +306,16 @@ __device__ __forceinline__ void dft_any(double2* v) {
   else dft_small<R, SIGN>(v);
 }
 
-// Orders LDS traffic of one wavefront: prior LDS writes of the wave are
-// visible to its later LDS reads and no access moves across this point.
+// Orders LDS traffic of one wavefront: prior LDS writes of the wave have
+// completed (s_waitcnt lgkmcnt(0)) before its later LDS reads, and neither the
+// IR optimiser (memory clobber) nor the machine scheduler (wave barrier) moves
+// an access across this point.  Lanes of one wave execute LDS instructions in
+// order, so no workgroup barrier is needed for wave-private LDS regions.
 __device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  asm volatile("" ::: "memory");
 }
 
 // LDS index with one pad slot per 16 elements (breaks the stride-16 pattern

@@ -223,7 +223,7 @@ int qs_solver_create(Mesh* mesh, const qs_params* params, qs_solver** out) {
 
 int qs_solver_destroy(qs_solver* S) {
   if (!S) return QS_OK;
-  if (S->mesh && S->mesh->priv) (void)hipStreamSynchronize(stream_of(S->mesh));
+  (void)hipDeviceSynchronize();   // the Mesh may already be gone: do not touch it
   for (int k = 0; k < 3; ++k) {
     if (S->old_[k]) (void)hipFree(S->old_[k]);
     if (S->new_[k]) (void)hipFree(S->new_[k]);
@@ -337,6 +337,38 @@ int qs_step_ifrk2_unfused(qs_solver* S, int nsteps) {
                             stream_of(m)));
   S->mixed_valid = false;
   return QS_OK;
+}
+
+// Test hooks: run one kernel of the fused step / read one internal array.
+int qs_solver_debug_phase(qs_solver* S, int phase) {
+  if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
+  switch (phase) {
+    case 0: QS_TRY(col_stage(S, MODE_PRO)); S->mixed_valid = true; return QS_OK;
+    case 1: return row_rhs(S);
+    case 2: return col_stage(S, MODE_S1);
+    case 3: return col_stage(S, MODE_S2);
+    default: return fail(QS_ERR_INVALID_ARG, "phase %d", phase);
+  }
+}
+
+int qs_solver_debug_array(qs_solver* S, int which, Qcomp* out) {
+  if (!S || !out) return fail(QS_ERR_INVALID_ARG, "null argument");
+  const Mesh* m = S->mesh;
+  const double2* src = nullptr;
+  if (which >= 0 && which < 7) src = S->mix[which];
+  else if (which >= 7 && which < 10) src = S->old_[which - 7];
+  else if (which >= 10 && which < 13) src = S->new_[which - 10];
+  else return fail(QS_ERR_INVALID_ARG, "array %d", which);
+  const size_t row = sizeof(Qcomp) * m->Nxh;
+  QS_HIP(hipMemcpy2DAsync(out, row, src, sizeof(double2) * S->pitch, row, m->Ny, hipMemcpyDeviceToDevice,
+                          stream_of(m)));
+  return QS_OK;
+}
+
+int qs_debug_wave_fft(const Mesh* mesh, int n, int sign, const Qcomp* in, Qcomp* out) {
+  QS_TRY(check_mesh(mesh));
+  if (!in || !out) return fail(QS_ERR_INVALID_ARG, "null argument");
+  return wave_fft_test(n, sign, (const double2*)in, (double2*)out, mesh->priv->tw1024, stream_of(mesh));
 }
 
 int qs_solver_profile(qs_solver* S, int enable) {

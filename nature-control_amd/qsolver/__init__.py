@@ -57,6 +57,23 @@ class QsParams(ctypes.Structure):
 
 _lib = None
 
+# Objects that own device memory, closed in dependency order at interpreter
+# exit (solvers, then fields, then meshes) before the HIP runtime goes away.
+import atexit  # noqa: E402
+import weakref  # noqa: E402
+
+_live = {"solver": weakref.WeakSet(), "field": weakref.WeakSet(), "mesh": weakref.WeakSet()}
+
+
+@atexit.register
+def _close_all():
+    for kind in ("solver", "field", "mesh"):
+        for obj in list(_live[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
 # name -> argtypes (all functions return int status unless listed in _RESTYPE)
 _SIGS = {
     "qs_mesh_create": [_I, _I, _D, _D, _I, ctypes.POINTER(ctypes.POINTER(MeshStruct))],
@@ -120,6 +137,9 @@ _SIGS = {
     "qs_step_ifrk2_unfused": [_P, _I],
     "qs_solver_nonfinite": [_P, ctypes.POINTER(_I)],
     "qs_solver_profile": [_P, _I],
+    "qs_solver_debug_phase": [_P, _I],
+    "qs_solver_debug_array": [_P, _I, _P],
+    "qs_debug_wave_fft": [_P, _I, _I, _P, _P],
     "qs_solver_profile_read": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I)],
     "qs_synth_uniform": [ctypes.c_uint64, _D, _D, ctypes.c_size_t, _P],
 }
@@ -189,6 +209,7 @@ class Mesh:
         Ly = Lx if Ly is None else Ly
         self._m = ctypes.POINTER(MeshStruct)()
         call("qs_mesh_create", Nx, Ny, float(Lx), float(Ly), BSZ, ctypes.byref(self._m))
+        _live["mesh"].add(self)
         self.attach_stream()
         s = self._m.contents
         self.Nx, self.Ny, self.Nxh = s.Nx, s.Ny, s.Nxh
@@ -254,6 +275,7 @@ class Field:
         self.mesh = mesh
         self._f = ctypes.POINTER(FieldStruct)()
         call("qs_field_create", mesh.handle, ctypes.byref(self._f))
+        _live["field"].add(self)
         self.spec = _view(self._f.contents.spec, mesh.spec_shape, torch.complex128, "<c16")
         self.phys = _view(self._f.contents.phys, mesh.phys_shape, torch.float64, "<f8")
 
@@ -294,6 +316,7 @@ class Solver:
         self.p = p
         self._s = ctypes.c_void_p()
         call("qs_solver_create", mesh.handle, ctypes.byref(p), ctypes.byref(self._s))
+        _live["solver"].add(self)
 
     def close(self):
         if getattr(self, "_s", None) and self._s.value:
@@ -335,6 +358,14 @@ class Solver:
         cnt = (_I * 4)()
         call("qs_solver_profile_read", self._s, avg, cnt)
         return {k: (avg[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
+
+    def debug_phase(self, phase):
+        call("qs_solver_debug_phase", self._s, int(phase))
+
+    def debug_array(self, which):
+        out = self.mesh.spec()
+        call("qs_solver_debug_array", self._s, int(which), ptr(out))
+        return out
 
     def nonfinite(self):
         flag = _I(0)
