@@ -51,81 +51,105 @@ struct TwN {
 // ===========================================================================
 // Row job
 // ===========================================================================
+
+// Z_k of a two-for-one C2R (Hermitian extensions of X and Y; imaginary parts
+// of the DC and Nyquist bins discarded).
+template <int N>
+__device__ __forceinline__ double2 herm_pair(int k, double2 X, double2 Y) {
+  if (k == 0 || 2 * k == N) return make_double2(X.x, Y.x);
+  if (2 * k < N) return make_double2(X.x - Y.y, X.y + Y.x);
+  return make_double2(X.x + Y.y, Y.x - X.y);
+}
+
+// Row job layout (one job = NV = 1024/N rows, 6 waves, 6 LDS buffers):
+//  1. the 7 mixed rows are loaded cooperatively (coalesced) into a staging
+//     view of the buffers;
+//  2. waves 0-4 build their two-for-one C2R inputs from the staging area,
+//     pairing fields of like magnitude: (w, dy u + dx v), (u, v), (r1, r2),
+//     (L1, L2), (dx u, -);
+//  3. inverse x FFTs (per wave) -> physical fields in buffers 0-4;
+//  4. pointwise products (all threads) -> buffers 0-5 as R2C pairs
+//     (A, B), (-u r1, -v r1), (-u r2, -v r2), (P11, P12), (P21, -), (loc1, loc2);
+//  5. forward x FFTs (per wave), spectra back into the buffers;
+//  6. cooperative unpack + spectral combination -> F0..F6 (coalesced stores).
 template <int N>
 __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm, const double2* twq) {
   constexpr int NV = 1024 / N;          // rows of this job (= vectors per wave FFT)
   constexpr int LPC = N / 16;
   constexpr int VS = N + N / 16;        // padded vector stride inside a wave buffer
+  constexpr int H = N / 2 + 1;
+  constexpr int RH = NV * H;            // staged bins per array
   const TwN<N> tw{twq};
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;     // lane = lp * NV + g
-  const int y = row0 + g;
-  const bool yok = y < a.Ny;
-  const size_t rowoff = (size_t)y * a.pitch;
   const double sc = a.scale;
-  double2* wb = sm + wave * kBuf;              // this wave's buffer
-  double2* vb = wb + g * VS;                   // this lane's vector
+  double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
 
-  // ---- x C2R of the nine fields (waves 0..4, one two-for-one FFT each) ----
-  if (wave < 5) {
-    const double w00 = *a.w00;
-    double2 v[16];
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int k = lp + LPC * m;
-      const bool hi = k > N / 2;
-      const int bin = hi ? N - k : k;
-      const double kx = a.kx1[bin];
-      double2 X = make_double2(0.0, 0.0), Y = make_double2(0.0, 0.0);
-      if (yok) {
-        const double2* M[7];
-#pragma unroll
-        for (int t = 0; t < 7; ++t) M[t] = a.mix[t] + rowoff + bin;
-        switch (wave) {
-          case 0: X = *M[0]; Y = *M[3]; break;                       // (w, r1)
-          case 1: X = *M[5]; Y = *M[2]; break;                       // (r2, u)
-          case 2: {                                                  // (v, dy u + dx v)
-            const double2 ps = *M[1], W = *M[0];
-            X = make_double2(kx * ps.y, -kx * ps.x);                 // v = -i kx psi
-            const double t2 = 2.0 * kx * kx;
-            Y = make_double2(-W.x + t2 * ps.x + (bin == 0 ? w00 : 0.0), -W.y + t2 * ps.y);
-          } break;
-          case 3: {                                                  // (L1, L2)
-            const double kx2 = kx * kx;
-            const double2 R1 = *M[3], R1yy = *M[4], R2 = *M[5], R2yy = *M[6];
-            X = make_double2(-kx2 * R1.x + R1yy.x, -kx2 * R1.y + R1yy.y);
-            Y = make_double2(-kx2 * R2.x + R2yy.x, -kx2 * R2.y + R2yy.y);
-          } break;
-          default: {                                                 // (dx u, -)
-            const double2 U = *M[2];
-            X = make_double2(-kx * U.y, kx * U.x);
-          } break;
-        }
-      }
-      if (bin == 0 || 2 * bin == N) v[m] = make_double2(X.x, Y.x);
-      else if (!hi) v[m] = make_double2(X.x - Y.y, X.y + Y.x);
-      else v[m] = make_double2(X.x + Y.y, Y.x - X.y);
-    }
-    wave_fft<N, +1>(v, vb, lp, tw);
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int x = lp + LPC * m;
-      vb[swz(x)] = make_double2(v[m].x * sc, v[m].y * sc);
-    }
+  // ---- 1. stage the mixed rows (array t, row vec, bin k) at t*RH + vec*H + k --
+  for (int e = tid; e < 7 * RH; e += 384) {
+    const int t = e / RH, r = e % RH, vec = r / H, k = r % H;
+    const int y = row0 + vec;
+    sm[e] = a.mix[t][(size_t)(y < a.Ny ? y : a.Ny - 1) * a.pitch + k];
   }
   __syncthreads();
 
-  // ---- pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) -
+  // ---- 2. C2R inputs --------------------------------------------------------
+  double2 v[16];
+  if (wave < 5) {
+    const double w00 = *a.w00;
+    const double2* st = sm + g * H;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int k = lp + LPC * m;
+      const int bin = k > N / 2 ? N - k : k;
+      const double kx = a.kx1[bin];
+      double2 X, Y;
+      if (wave == 0) {            // (w, dy u + dx v = -W + 2 kx^2 psi + W00 [kx = 0])
+        const double2 W = st[0 * RH + bin], ps = st[1 * RH + bin];
+        const double t2 = 2.0 * kx * kx;
+        X = W;
+        Y = make_double2(-W.x + t2 * ps.x + (bin == 0 ? w00 : 0.0), -W.y + t2 * ps.y);
+      } else if (wave == 1) {     // (u = i ky psi, v = -i kx psi)
+        const double2 ps = st[1 * RH + bin];
+        X = st[2 * RH + bin];
+        Y = make_double2(kx * ps.y, -kx * ps.x);
+      } else if (wave == 2) {     // (r1, r2)
+        X = st[3 * RH + bin];
+        Y = st[5 * RH + bin];
+      } else if (wave == 3) {     // (L1, L2) = -kx^2 R + R_yy
+        const double2 R1 = st[3 * RH + bin], R1yy = st[4 * RH + bin];
+        const double2 R2 = st[5 * RH + bin], R2yy = st[6 * RH + bin];
+        X = make_double2(-kx * kx * R1.x + R1yy.x, -kx * kx * R1.y + R1yy.y);
+        Y = make_double2(-kx * kx * R2.x + R2yy.x, -kx * kx * R2.y + R2yy.y);
+      } else {                    // (dx u = i kx u, -)
+        const double2 U = st[2 * RH + bin];
+        X = make_double2(-kx * U.y, kx * U.x);
+        Y = make_double2(0.0, 0.0);
+      }
+      v[m] = herm_pair<N>(k, X, Y);
+    }
+  }
+  __syncthreads();   // staging area free
+
+  // ---- 3. inverse x FFTs -> physical fields ------------------------------------
+  if (wave < 5) {
+    wave_fft<N, +1>(v, vb, lp, tw);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = make_double2(v[m].x * sc, v[m].y * sc);
+  }
+  __syncthreads();
+
+  // ---- 4. pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) --
   for (int p = tid; p < 1024; p += 384) {
     const int gg = p / N, x = p % N;
     const int o = gg * VS + swz(x);
     const int yy = row0 + gg;
     const double2 f0 = sm[0 * kBuf + o], f1 = sm[1 * kBuf + o], f2 = sm[2 * kBuf + o],
                   f3 = sm[3 * kBuf + o], f4 = sm[4 * kBuf + o];
-    const double w = f0.x, r1 = f0.y, r2 = f1.x, u = f1.y, v = f2.x, uyvx = f2.y;
+    const double w = f0.x, uyvx = f0.y, u = f1.x, v = f1.y, r1 = f2.x, r2 = f2.y;
     const double L1 = f3.x, L2 = f3.y, ux = f4.x;
-    const double al = a.Ra ? (yy < a.Ny ? a.Ra[(size_t)yy * N + x] : 0.0) : a.alpha;
+    const double al = a.Ra ? a.Ra[(size_t)(yy < a.Ny ? yy : a.Ny - 1) * N + x] : a.alpha;
     const double S = 2 * sqrt(r1 * r1 + r2 * r2);
     const double cS2 = a.cQ * S * S;
     const double gg1 = a.lambda * S * (cS2 - a.aQ);
@@ -140,75 +164,70 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
     const double pB = a.advect_w ? -1.0 * v * w : 0.0;
     const double loc1 = (-1.0 * w * r2 - cS2 * r1) + a.lambda * S * ux;
     const double loc2 = (1.0 * w * r1 - cS2 * r2) + 0.5 * a.lambda * S * uyvx;
-    sm[0 * kBuf + o] = make_double2(pA, p12);
-    sm[1 * kBuf + o] = make_double2(pB, p11);
-    sm[2 * kBuf + o] = make_double2(p21, 0.0);
-    sm[3 * kBuf + o] = make_double2(loc1, -1.0 * u * r1);
-    sm[4 * kBuf + o] = make_double2(loc2, -1.0 * u * r2);
-    sm[5 * kBuf + o] = make_double2(-1.0 * v * r1, -1.0 * v * r2);
+    sm[0 * kBuf + o] = make_double2(pA, pB);
+    sm[1 * kBuf + o] = make_double2(-1.0 * u * r1, -1.0 * v * r1);
+    sm[2 * kBuf + o] = make_double2(-1.0 * u * r2, -1.0 * v * r2);
+    sm[3 * kBuf + o] = make_double2(p11, p12);
+    sm[4 * kBuf + o] = make_double2(p21, 0.0);
+    sm[5 * kBuf + o] = make_double2(loc1, loc2);
   }
   __syncthreads();
 
-  // ---- x R2C of the products (waves 0..5) and the spectral combinations ---
-  {
-    double2 v[16];
+  // ---- 5. forward x FFTs ------------------------------------------------------
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = vb[swz(lp + LPC * m)];
-    wave_fft<N, -1>(v, vb, lp, tw);
+  for (int m = 0; m < 16; ++m) v[m] = vb[swz(lp + LPC * m)];
+  wave_sync();
+  wave_fft<N, -1>(v, vb, lp, tw);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = v[m];
-    wave_sync();
-    if (yok) {
-      const double sE = a.s_ErRe;
+  for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = v[m];
+  __syncthreads();
+
+  // ---- 6. unpack the two-for-one spectra and form F0..F6 ---------------------
+  const double sE = a.s_ErRe;
+  for (int e = tid; e < RH; e += 384) {
+    const int vec = e / H, k = e % H;
+    const int y = row0 + vec;
+    if (y >= a.Ny) continue;
+    const int o = vec * VS + swz(k), oc = vec * VS + swz((N - k) & (N - 1));
+    double2 Sp[6][2];
 #pragma unroll
-      for (int m = 0; m < 9; ++m) {
-        const int k = lp + LPC * m;
-        if (k <= N / 2) {
-          const double2 zk = vb[swz(k)];
-          const double2 zc = cconj(vb[swz((N - k) & (N - 1))]);
-          const double2 A = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
-          const double2 d = csub(zk, zc);
-          const double2 B = make_double2(0.5 * d.y, -0.5 * d.x);
-          const double kx = a.kx1[k];
-          const size_t o = rowoff + k;
-          switch (wave) {
-            case 0: {   // (A, P12) -> F0
-              const double kxa = (2 * k == N) ? 0.0 : kx;
-              const double c = -kx * kx * sE;
-              a.mix[0][o] = make_double2(-kxa * A.y + c * B.x, kxa * A.x + c * B.y);
-            } break;
-            case 1: {   // (B, P11) -> F1
-              const double t = 2.0 * kx * sE;
-              a.mix[1][o] = make_double2(A.x + t * B.y, A.y - t * B.x);
-            } break;
-            case 2:     // (P21, -) -> F2
-              a.mix[2][o] = make_double2(sE * A.x, sE * A.y);
-              break;
-            case 3:     // (loc1, -u r1) -> F3
-              a.mix[3][o] = make_double2(A.x - kx * B.y, A.y + kx * B.x);
-              break;
-            case 4:     // (loc2, -u r2) -> F5
-              a.mix[5][o] = make_double2(A.x - kx * B.y, A.y + kx * B.x);
-              break;
-            default:    // (-v r1, -v r2) -> F4, F6
-              a.mix[4][o] = A;
-              a.mix[6][o] = B;
-              break;
-          }
-        }
-      }
+    for (int q = 0; q < 6; ++q) {
+      const double2 zk = sm[q * kBuf + o], zc = cconj(sm[q * kBuf + oc]);
+      Sp[q][0] = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+      const double2 d = csub(zk, zc);
+      Sp[q][1] = make_double2(0.5 * d.y, -0.5 * d.x);
     }
+    const double2 A = Sp[0][0], B = Sp[0][1], U1 = Sp[1][0], V1 = Sp[1][1], U2 = Sp[2][0], V2 = Sp[2][1];
+    const double2 P11 = Sp[3][0], P12 = Sp[3][1], P21 = Sp[4][0], Lc1 = Sp[5][0], Lc2 = Sp[5][1];
+    const double kx = a.kx1[k];
+    const double kxa = (2 * k == N) ? 0.0 : kx;      // projected x-derivative of the advection
+    const double c12 = -kx * kx * sE, t11 = 2.0 * kx * sE;
+    const size_t ro = (size_t)y * a.pitch + k;
+    a.mix[0][ro] = make_double2(-kxa * A.y + c12 * P12.x, kxa * A.x + c12 * P12.y);  // i kx A - kx^2 s P12
+    a.mix[1][ro] = make_double2(B.x + t11 * P11.y, B.y - t11 * P11.x);             // B - 2 i kx s P11
+    a.mix[2][ro] = make_double2(sE * P21.x, sE * P21.y);                             // s P21
+    a.mix[3][ro] = make_double2(Lc1.x - kx * U1.y, Lc1.y + kx * U1.x);               // loc1 + i kx U1
+    a.mix[4][ro] = V1;
+    a.mix[5][ro] = make_double2(Lc2.x - kx * U2.y, Lc2.y + kx * U2.x);               // loc2 + i kx U2
+    a.mix[6][ro] = V2;
   }
 }
 
 // ===========================================================================
 // Column job
 // ===========================================================================
+// Tile of TC = 4 * 1024/NY columns: wave w owns columns i0 + w*CW + g (g <
+// CW = 1024/NY).  Global traffic is staged through the wave buffers so that
+// every load / store instruction covers whole row segments of the tile
+// (coalesced), while the FFTs run per wavefront on the lane layout
+// "lane (lp, g) holds rows lp + LPC*m".
 template <int NY, int MODE>
-__device__ __forceinline__ void col_job(const RunDesc& a, int fld, int i0, double2* sm, const double2* twq) {
+__device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq) {
   constexpr int CW = 1024 / NY;          // columns per wave
+  constexpr int TC = 4 * CW;             // columns per tile
   constexpr int LPC = NY / 16;
   constexpr int VS = NY + NY / 16;
+  constexpr int WS = kBuf + 1;           // wave buffer stride (odd: spreads LDS banks)
   const TwN<NY> tw{twq};
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -216,107 +235,165 @@ __device__ __forceinline__ void col_job(const RunDesc& a, int fld, int i0, doubl
   const int i = i0 + wave * CW + g;
   const bool iok = i < a.Nxh;
   const int ic = iok ? i : 0;
-  double2* vb = sm + wave * kBuf + g * VS;
+  double2* vb = sm + wave * WS + g * VS;
   const double kx = a.kx1[ic];
   const int nyq_x = a.Nx / 2;
+  const size_t pitch = a.pitch;
   bool bad = false;
+
+  // tile element (row j, column c) <-> LDS slot
+  auto slot = [&](int j, int c) { return (c / CW) * WS + (c % CW) * VS + swz(j); };
+  // cooperative coalesced tile load into the wave buffers (ends with a barrier)
+  auto tile_load = [&](const double2* __restrict__ src) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q;
+      const int c = e % TC, j = e / TC;
+      const int ii = i0 + c;
+      const double2 val = src[(size_t)j * pitch + (ii < a.Nxh ? ii : a.Nxh - 1)];
+      sm[slot(j, c)] = ii < a.Nxh ? val : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+  };
+  // cooperative coalesced tile store from the wave buffers
+  auto tile_store = [&](double2* __restrict__ dst) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q;
+      const int c = e % TC, j = e / TC;
+      const int ii = i0 + c;
+      if (ii < a.Nxh) dst[(size_t)j * pitch + ii] = sm[slot(j, c)];
+    }
+  };
+  auto stage_in = [&](const double2* __restrict__ src, double2 (&v)[16]) {
+    tile_load(src);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = vb[swz(lp + LPC * m)];
+    wave_sync();
+  };
+  auto stage_out = [&](double2* __restrict__ dst, const double2 (&v)[16]) {
+#pragma unroll
+    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = v[m];
+    tile_store(dst);
+  };
 
   double2 st[16];   // the state of the next stage (curr / new / old)
   if (MODE == MODE_PRO) {
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int j = lp + LPC * m;
-      st[m] = iok ? a.old_[fld][(size_t)j * a.pitch + i] : make_double2(0.0, 0.0);
-    }
+    stage_in(a.old_[FLD], st);
   } else {
-    // ---- N_X from the forward set ----------------------------------------
-    double2 nl[16];
-    const int f0 = fld == 0 ? 0 : (fld == 1 ? 3 : 5);
-    const int nterm = fld == 0 ? 3 : 2;
-#pragma unroll 1
-    for (int t = 0; t < nterm; ++t) {
+    // ---- N_X from the forward set ------------------------------------------
+    //   N_w = F0 + i ky F1 + ky^2 F2,  N_r = F_a + i ky F_b   (in st)
+    const int f0 = FLD == 0 ? 0 : (FLD == 1 ? 3 : 5);
+    stage_in(a.mix[f0], st);
+    wave_fft<NY, -1>(st, vb, lp, tw);
+    {
       double2 v[16];
-      const double2* src = a.mix[f0 + t];
-#pragma unroll
-      for (int m = 0; m < 16; ++m) {
-        const int j = lp + LPC * m;
-        v[m] = iok ? src[(size_t)j * a.pitch + i] : make_double2(0.0, 0.0);
-      }
+      stage_in(a.mix[f0 + 1], v);
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
         const double ky = a.ky1[lp + LPC * m];
-        const double2 z = v[m];
-        if (t == 0) nl[m] = z;
-        else if (t == 1) nl[m] = make_double2(nl[m].x - ky * z.y, nl[m].y + ky * z.x);
-        else nl[m] = make_double2(nl[m].x + ky * ky * z.x, nl[m].y + ky * ky * z.y);
+        st[m] = make_double2(st[m].x - ky * v[m].y, st[m].y + ky * v[m].x);
       }
     }
-    // ---- IFRK2 stage update -------------------------------------------------
+    if (FLD == 0) {
+      double2 v[16];
+      stage_in(a.mix[2], v);
+      wave_fft<NY, -1>(v, vb, lp, tw);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const double ky = a.ky1[lp + LPC * m];
+        st[m] = make_double2(st[m].x + ky * ky * v[m].x, st[m].y + ky * ky * v[m].y);
+      }
+    }
+    // ---- IFRK2 stage update (state tile read and rewritten in LDS) ---------------
+    tile_load(MODE == MODE_S1 ? a.old_[FLD] : a.new_[FLD]);
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
       const int j = lp + LPC * m;
-      if (!iok) { st[m] = make_double2(0.0, 0.0); continue; }
-      double2 n = nl[m];
-      if (fld == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) n.y = 0.0;
-      const size_t o = (size_t)j * a.pitch + i;
+      double2* sl = vb + swz(j);
+      const double2 in = *sl;
+      double2 n = st[m];
+      if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) n.y = 0.0;
       const double ky = a.ky1[j];
       const double k2 = kx * kx + ky * ky;
-      const double Lk = fld == 0 ? 1.0 / a.Re * (-1.0 * k2 - a.Rf) : (-1.0 * a.DQ * k2 + a.aQ);
+      const double Lk = FLD == 0 ? 1.0 / a.Re * (-1.0 * k2 - a.Rf) : (-1.0 * a.DQ * k2 + a.aQ);
       const double IF = exp(Lk * a.dt);
       const double2 an = make_double2(n.x * a.dt, n.y * a.dt);
       if (MODE == MODE_S1) {
-        const double2 old = a.old_[fld][o];
+        const double2 old = in;
         const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // IFRK2.cuh:16-26
         const double h = 1.0 / 2.0 * IF;
-        a.new_[fld][o] = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);     // IFRK2.cuh:28-39
+        *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // IFRK2.cuh:28-39
         st[m] = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);      // curr (IFh = IF)
       } else {
-        const double2 nw = a.new_[fld][o];
+        const double2 nw = in;
         double2 r = make_double2(nw.x + 1.0 / 2.0 * an.x, nw.y + 1.0 / 2.0 * an.y);  // IFRK2.cuh:41-51
         const bool keep = (i < ((a.Nxh - 1) * 2 / 3 + 1)) && ((j < NY / 3 + 1) || ((NY - j) < NY / 3));
         const double cut = keep ? 1.0 : 0.0;                                  // Qsolver.cuh:314-322
         r = make_double2(r.x * cut, r.y * cut);
-        bad |= !isfinite(r.x) || !isfinite(r.y);
-        a.old_[fld][o] = r;                                                   // next state
+        if (iok) bad |= !isfinite(r.x) || !isfinite(r.y);
+        *sl = r;
         st[m] = r;
       }
     }
+    tile_store(MODE == MODE_S1 ? a.new_[FLD] : a.old_[FLD]);   // stage 2: next state over old
   }
 
-  // ---- inverse set of the next stage from st --------------------------------
-  const int nout = fld == 0 ? 3 : 2;
-  const int m0 = fld == 0 ? 0 : (fld == 1 ? 3 : 5);
-  if (fld == 0 && i == 0 && lp == 0 && a.w00) *a.w00 = st[0].x;   // element (j=0, i=0)
-#pragma unroll 1
-  for (int t = 0; t < nout; ++t) {
-    double2 v[16];
+  // ---- inverse set of the next stage from st ----------------------------------
+  if (FLD == 0) {
+    // M0 = W, M1 = psi = W/k^2, M2 = i ky psi   (vel_func, QFldfuncs.cuh:204-224)
+    if (i == 0 && lp == 0 && a.w00) *a.w00 = st[0].x;   // element (j=0, i=0)
+    {
+      double2 v[16];
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < 16; ++m) v[m] = st[m];
+      wave_fft<NY, +1>(v, vb, lp, tw);
+      stage_out(a.mix[0], v);
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {   // st <- psi
       const int j = lp + LPC * m;
       const double ky = a.ky1[j];
-      const double2 s = st[m];
-      if (fld == 0) {   // W, psi = W/k^2, i ky psi  (vel_func, QFldfuncs.cuh:204-224)
-        if (t == 0) {
-          v[m] = s;
-        } else {
-          const double k2 = kx * kx + ky * ky;
-          const double2 ps = (i == 0 && j == 0) ? make_double2(0.0, 0.0) : make_double2(s.x / k2, s.y / k2);
-          v[m] = t == 1 ? ps : make_double2(-ky * ps.y, ky * ps.x);
-        }
-      } else {          // R, -ky^2 R
-        v[m] = t == 0 ? s : make_double2(-ky * ky * s.x, -ky * ky * s.y);
-      }
+      const double k2 = kx * kx + ky * ky;
+      st[m] = (i == 0 && j == 0) || !iok ? make_double2(0.0, 0.0) : make_double2(st[m].x / k2, st[m].y / k2);
     }
-    wave_fft<NY, +1>(v, vb, lp, tw);
-    if (iok) {
-      double2* dst = a.mix[m0 + t];
+    {
+      double2 v[16];
 #pragma unroll
-      for (int m = 0; m < 16; ++m) dst[(size_t)(lp + LPC * m) * a.pitch + i] = v[m];
+      for (int m = 0; m < 16; ++m) v[m] = st[m];
+      wave_fft<NY, +1>(v, vb, lp, tw);
+      stage_out(a.mix[1], v);
     }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {   // st <- i ky psi
+      const double ky = a.ky1[lp + LPC * m];
+      st[m] = make_double2(-ky * st[m].y, ky * st[m].x);
+    }
+    wave_fft<NY, +1>(st, vb, lp, tw);
+    stage_out(a.mix[2], st);
+  } else {
+    const int m0 = FLD == 1 ? 3 : 5;
+    {
+      double2 v[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) v[m] = st[m];
+      wave_fft<NY, +1>(v, vb, lp, tw);
+      stage_out(a.mix[m0], v);
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {   // st <- -ky^2 R
+      const double ky = a.ky1[lp + LPC * m];
+      st[m] = make_double2(-ky * ky * st[m].x, -ky * ky * st[m].y);
+    }
+    wave_fft<NY, +1>(st, vb, lp, tw);
+    stage_out(a.mix[m0 + 1], st);
   }
   if (MODE == MODE_S2 && bad) atomicOr(a.flag, 1);
 }
+
 
 // ===========================================================================
 // Kernels: one run (grid over jobs) and an ensemble (work list over runs).
@@ -386,7 +463,7 @@ __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double
 template <int NFIX, int MODE>
 __global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
                                              int nwork, const double2* __restrict__ tw1024) {
-  __shared__ double2 sm[4 * kBuf];
+  __shared__ double2 sm[4 * (kBuf + 1)];
   __shared__ double2 twq[kTwq];
   stage_twiddles(twq, tw1024, threadIdx.x, 256);
   __syncthreads();

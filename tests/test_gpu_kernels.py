@@ -52,8 +52,9 @@ def test_fused_phases(N):
         assert rel(a, b) < 1e-14
     g.debug_phase(1)
     F = [g.debug_array(k).cpu().numpy() for k in range(7)]
+    # white-noise inputs: the k^2-weighted products amplify roundoff
     for a, b in zip(F, FR.row_pass(M, st[0][0, 0], P, N, N, kx)):
-        assert rel(a, b) < 1e-13
+        assert rel(a, b) < 1e-12
     g.debug_phase(2)
     Nw, Nr1, Nr2 = FR.nonlinear_from_forward_set(F, kx, ky, N, N)
     k2 = kx**2 + ky**2
@@ -61,8 +62,8 @@ def test_fused_phases(N):
     IFq = np.exp((-P["D_Q"] * k2 + P["a_Q"]) * dt)
     for k, (old, n, IF) in enumerate(zip(st, (Nw, Nr1, Nr2), (IFw, IFq, IFq))):
         want = old * IF + 0.5 * IF * dt * n
-        assert rel(g.debug_array(10 + k).cpu().numpy(), want) < 1e-13
+        assert rel(g.debug_array(10 + k).cpu().numpy(), want) < 1e-12
     Mc = [g.debug_array(k).cpu().numpy() for k in range(7)]
     curr = [(old + dt * n) * IF for old, n, IF in zip(st, (Nw, Nr1, Nr2), (IFw, IFq, IFq))]
     for a, b in zip(Mc, FR.inverse_set(curr[0], curr[1], curr[2], kx, ky)):
-        assert rel(a, b) < 1e-13
+        assert rel(a, b) < 1e-12
