@@ -87,23 +87,43 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
   double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
 
   // ---- 1. stage the mixed rows (array t, row vec, bin k) at t*RH + vec*H + k --
-  for (int e = tid; e < 7 * RH; e += 384) {
-    const int t = e / RH, r = e % RH, vec = r / H, k = r % H;
-    const int y = row0 + vec;
-    sm[e] = a.mix[t][(size_t)(y < a.Ny ? y : a.Ny - 1) * a.pitch + k];
+  // RH = 512 + NV <= 576 < 2 * 384: two slots per thread and array, all 14
+  // loads in flight before the first LDS write.
+  static_assert(RH <= 2 * 384, "row staging assumes two slots per thread");
+  {
+    double2 ld[7][2];
+    size_t off[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = tid + 384 * h;
+      const int rr = r < RH ? r : RH - 1;
+      const int vec = rr / H, k = rr % H;
+      const int y = row0 + vec;
+      off[h] = (size_t)(y < a.Ny ? y : a.Ny - 1) * a.pitch + k;
+    }
+#pragma unroll
+    for (int t = 0; t < 7; ++t)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) ld[t][h] = gload(a.mix[t] + off[h]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < 7; ++t)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (tid + 384 * h < RH) sm[t * RH + tid + 384 * h] = ld[t][h];
   }
   __syncthreads();
 
   // ---- 2. C2R inputs --------------------------------------------------------
   double2 v[16];
   if (wave < 5) {
-    const double w00 = *a.w00;
+    const double w00 = gload(a.w00);
     const double2* st = sm + g * H;
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
       const int k = lp + LPC * m;
       const int bin = k > N / 2 ? N - k : k;
-      const double kx = a.kx1[bin];
+      const double kx = a.dkx * bin;
       double2 X, Y;
       if (wave == 0) {            // (w, dy u + dx v = -W + 2 kx^2 psi + W00 [kx = 0])
         const double2 W = st[0 * RH + bin], ps = st[1 * RH + bin];
@@ -149,7 +169,7 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
                   f3 = sm[3 * kBuf + o], f4 = sm[4 * kBuf + o];
     const double w = f0.x, uyvx = f0.y, u = f1.x, v = f1.y, r1 = f2.x, r2 = f2.y;
     const double L1 = f3.x, L2 = f3.y, ux = f4.x;
-    const double al = a.Ra ? a.Ra[(size_t)(yy < a.Ny ? yy : a.Ny - 1) * N + x] : a.alpha;
+    const double al = a.Ra ? gload(a.Ra + (size_t)(yy < a.Ny ? yy : a.Ny - 1) * N + x) : a.alpha;
     const double S = 2 * sqrt(r1 * r1 + r2 * r2);
     const double cS2 = a.cQ * S * S;
     const double gg1 = a.lambda * S * (cS2 - a.aQ);
@@ -199,17 +219,17 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
     }
     const double2 A = Sp[0][0], B = Sp[0][1], U1 = Sp[1][0], V1 = Sp[1][1], U2 = Sp[2][0], V2 = Sp[2][1];
     const double2 P11 = Sp[3][0], P12 = Sp[3][1], P21 = Sp[4][0], Lc1 = Sp[5][0], Lc2 = Sp[5][1];
-    const double kx = a.kx1[k];
+    const double kx = a.dkx * k;
     const double kxa = (2 * k == N) ? 0.0 : kx;      // projected x-derivative of the advection
     const double c12 = -kx * kx * sE, t11 = 2.0 * kx * sE;
     const size_t ro = (size_t)y * a.pitch + k;
-    a.mix[0][ro] = make_double2(-kxa * A.y + c12 * P12.x, kxa * A.x + c12 * P12.y);  // i kx A - kx^2 s P12
-    a.mix[1][ro] = make_double2(B.x + t11 * P11.y, B.y - t11 * P11.x);             // B - 2 i kx s P11
-    a.mix[2][ro] = make_double2(sE * P21.x, sE * P21.y);                             // s P21
-    a.mix[3][ro] = make_double2(Lc1.x - kx * U1.y, Lc1.y + kx * U1.x);               // loc1 + i kx U1
-    a.mix[4][ro] = V1;
-    a.mix[5][ro] = make_double2(Lc2.x - kx * U2.y, Lc2.y + kx * U2.x);               // loc2 + i kx U2
-    a.mix[6][ro] = V2;
+    gstore(a.mix[0] + ro, make_double2(-kxa * A.y + c12 * P12.x, kxa * A.x + c12 * P12.y));  // i kx A - kx^2 s P12
+    gstore(a.mix[1] + ro, make_double2(B.x + t11 * P11.y, B.y - t11 * P11.x));             // B - 2 i kx s P11
+    gstore(a.mix[2] + ro, make_double2(sE * P21.x, sE * P21.y));                             // s P21
+    gstore(a.mix[3] + ro, make_double2(Lc1.x - kx * U1.y, Lc1.y + kx * U1.x));               // loc1 + i kx U1
+    gstore(a.mix[4] + ro, V1);
+    gstore(a.mix[5] + ro, make_double2(Lc2.x - kx * U2.y, Lc2.y + kx * U2.x));               // loc2 + i kx U2
+    gstore(a.mix[6] + ro, V2);
   }
 }
 
@@ -236,7 +256,9 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   const bool iok = i < a.Nxh;
   const int ic = iok ? i : 0;
   double2* vb = sm + wave * WS + g * VS;
-  const double kx = a.kx1[ic];
+  const double kx = a.dkx * ic;
+  // ky(j) of the mesh tables (Basic/Mesh.cuh:6-21)
+  auto kyj = [&](int j) { return a.dky * (j < NY / 2 + 1 ? j : j - NY); };
   const int nyq_x = a.Nx / 2;
   const size_t pitch = a.pitch;
   bool bad = false;
@@ -245,14 +267,23 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   auto slot = [&](int j, int c) { return (c / CW) * WS + (c % CW) * VS + swz(j); };
   // cooperative coalesced tile load into the wave buffers (ends with a barrier)
   auto tile_load = [&](const double2* __restrict__ src) {
-    __syncthreads();
+    // all 16 loads in flight before the first LDS write (the scheduling
+    // barrier keeps the compiler from pairing each load with its write)
+    double2 ld[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int e = tid + 256 * q;
       const int c = e % TC, j = e / TC;
       const int ii = i0 + c;
-      const double2 val = src[(size_t)j * pitch + (ii < a.Nxh ? ii : a.Nxh - 1)];
-      sm[slot(j, c)] = ii < a.Nxh ? val : make_double2(0.0, 0.0);
+      ld[q] = gload(src + (size_t)j * pitch + (ii < a.Nxh ? ii : a.Nxh - 1));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q;
+      const int c = e % TC, j = e / TC;
+      sm[slot(j, c)] = i0 + c < a.Nxh ? ld[q] : make_double2(0.0, 0.0);
     }
     __syncthreads();
   };
@@ -264,7 +295,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const int e = tid + 256 * q;
       const int c = e % TC, j = e / TC;
       const int ii = i0 + c;
-      if (ii < a.Nxh) dst[(size_t)j * pitch + ii] = sm[slot(j, c)];
+      if (ii < a.Nxh) gstore(dst + (size_t)j * pitch + ii, sm[slot(j, c)]);
     }
   };
   auto stage_in = [&](const double2* __restrict__ src, double2 (&v)[16]) {
@@ -294,7 +325,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
-        const double ky = a.ky1[lp + LPC * m];
+        const double ky = kyj(lp + LPC * m);
         st[m] = make_double2(st[m].x - ky * v[m].y, st[m].y + ky * v[m].x);
       }
     }
@@ -304,7 +335,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
-        const double ky = a.ky1[lp + LPC * m];
+        const double ky = kyj(lp + LPC * m);
         st[m] = make_double2(st[m].x + ky * ky * v[m].x, st[m].y + ky * ky * v[m].y);
       }
     }
@@ -317,7 +348,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const double2 in = *sl;
       double2 n = st[m];
       if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) n.y = 0.0;
-      const double ky = a.ky1[j];
+      const double ky = kyj(j);
       const double k2 = kx * kx + ky * ky;
       const double Lk = FLD == 0 ? 1.0 / a.Re * (-1.0 * k2 - a.Rf) : (-1.0 * a.DQ * k2 + a.aQ);
       const double IF = exp(Lk * a.dt);
@@ -345,7 +376,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   // ---- inverse set of the next stage from st ----------------------------------
   if (FLD == 0) {
     // M0 = W, M1 = psi = W/k^2, M2 = i ky psi   (vel_func, QFldfuncs.cuh:204-224)
-    if (i == 0 && lp == 0 && a.w00) *a.w00 = st[0].x;   // element (j=0, i=0)
+    if (i == 0 && lp == 0 && a.w00) *(QS_GLOBAL double*)a.w00 = st[0].x;   // element (j=0, i=0)
     {
       double2 v[16];
 #pragma unroll
@@ -356,7 +387,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
 #pragma unroll
     for (int m = 0; m < 16; ++m) {   // st <- psi
       const int j = lp + LPC * m;
-      const double ky = a.ky1[j];
+      const double ky = kyj(j);
       const double k2 = kx * kx + ky * ky;
       st[m] = (i == 0 && j == 0) || !iok ? make_double2(0.0, 0.0) : make_double2(st[m].x / k2, st[m].y / k2);
     }
@@ -369,7 +400,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) {   // st <- i ky psi
-      const double ky = a.ky1[lp + LPC * m];
+      const double ky = kyj(lp + LPC * m);
       st[m] = make_double2(-ky * st[m].y, ky * st[m].x);
     }
     wave_fft<NY, +1>(st, vb, lp, tw);
@@ -385,7 +416,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) {   // st <- -ky^2 R
-      const double ky = a.ky1[lp + LPC * m];
+      const double ky = kyj(lp + LPC * m);
       st[m] = make_double2(-ky * ky * st[m].x, -ky * ky * st[m].y);
     }
     wave_fft<NY, +1>(st, vb, lp, tw);

@@ -16,8 +16,7 @@ struct RunDesc {
   const double* Ra;               // activity field [Ny][Nx] or null (uniform alpha)
   double* w00;                    // W(0,0) of the current stage
   int* flag;                      // non-finite flag
-  const double* kx1;              // kx(i), i < Nxh
-  const double* ky1;              // ky(j), j < Ny
+  double dkx, dky;                // 2 pi / Lx, 2 pi / Ly: kx(i) = dkx i, ky(j) = dky (j or j - Ny)
   double alpha, lambda, DQ, aQ, cQ, s_ErRe, scale, dt, Re, Rf;
   int advect_w;
 };

@@ -257,6 +257,22 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
   return x * per + (x < rem ? x : rem) + slot;
 }
 
+// Global-memory accessors.  Pointers read from a descriptor are generic to the
+// compiler, which then emits flat instructions: they count against both the
+// vector-memory and the LDS counters, so every LDS wait also waits for them.
+// Casting to the global address space gives global_load / global_store.
+#define QS_GLOBAL __attribute__((address_space(1)))
+typedef double qs_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 gload(const double2* p) {
+  const qs_d2v v = *(const QS_GLOBAL qs_d2v*)p;
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ double gload(const double* p) { return *(const QS_GLOBAL double*)p; }
+__device__ __forceinline__ void gstore(double2* p, double2 v) {
+  const qs_d2v t = {v.x, v.y};
+  *(QS_GLOBAL qs_d2v*)p = t;
+}
+
 }  // namespace qs
 
 namespace qs {

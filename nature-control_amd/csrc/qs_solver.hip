@@ -56,8 +56,8 @@ RunDesc make_desc(qs_solver* S) {
   d.Ra = S->Ra;
   d.w00 = S->w00;
   d.flag = S->flag;
-  d.kx1 = m->priv->kx1;
-  d.ky1 = m->priv->ky1;
+  d.dkx = 2 * M_PI / m->Lx;   // the mesh tables' own expressions (Basic/Mesh.cuh:6-21)
+  d.dky = 2 * M_PI / m->Ly;
   d.alpha = S->p.alpha; d.lambda = S->p.lambda;
   d.DQ = S->p.D_Q; d.aQ = S->p.a_Q; d.cQ = S->p.c_Q;
   d.s_ErRe = 1.0 / (S->p.Er * S->p.Re);
