@@ -38,7 +38,43 @@ def parse():
                     help="reload the initial state every R steps (inside the timed region)")
     ap.add_argument("--unfused", action="store_true",
                     help="time the per-op (reference call structure) step instead")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times "
+                         "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_calibrate(mib):
+    import torch
+    n = mib * 2**20 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}), flush=True)
+
+
+def traffic_from_profile(kernel, N):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/fused_kernels_<N>_traffic.md: rocprofv3 FETCH_SIZE and WRITE_SIZE
+    passes scaled by a plain-copy calibration), used only when it was measured
+    on the current kernel source (its sha256 is recorded in the summary)."""
+    import hashlib
+    path = os.path.join(ROOT, "profiles", f"fused_kernels_{N}_traffic.md")
+    src = os.path.join(ROOT, "nature-control_amd", "csrc", "qs_fused.hip")
+    if not (os.path.exists(path) and os.path.exists(src)):
+        return None
+    sha = hashlib.sha256(open(src, "rb").read()).hexdigest()
+    vals = {}
+    for line in open(path):
+        if line.startswith("- ") and ": " in line:
+            k, v = line[2:].split(": ", 1)
+            vals[k.strip("` ")] = v.strip().strip("`")
+    key = f"{kernel}.traffic_bytes_per_launch"
+    if vals.get("source_sha256") != sha or key not in vals:
+        return None
+    return int(float(vals[key]))
 
 
 def cpu_baseline(N, dt, nsteps, q11, q12):
@@ -66,9 +102,12 @@ def cpu_baseline(N, dt, nsteps, q11, q12):
 
 def main():
     args = parse()
+    if args.copy_calibrate:
+        copy_calibrate(args.copy_calibrate)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the ensemble and batched adjoint shard by run with no exchange; see DESIGN.md)",
+              "(the ensemble and batched adjoint shard by run with no exchange)",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
@@ -135,13 +174,14 @@ def main():
     if dom:
         ach = kernels[dom]["GBps"]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_from_profile(dom, N)}
     step_bytes = 68 * H
     cpu = None
     if args.cpu_steps > 0:
         cpu = cpu_baseline(N, dt, args.cpu_steps, q11, q12)
     out = {
-        "metric": "IFRK2 steps/s at 1024^2 FP64",
+        "metric": "IFRK2 steps/s at 1024\u00b2 FP64 + adjoint grad-steps/s; % of HBM roofline",
         "value": round(steps_per_s, 2),
         "unit": "steps/s",
         "n_gpus": 1,
