@@ -188,7 +188,8 @@ int qs_solver_set_activity(qs_solver* s, const Qreal* Ra);
 int qs_solver_set_state(qs_solver* s, const Qcomp* w, const Qcomp* r1, const Qcomp* r2);
 int qs_solver_get_state(qs_solver* s, Qcomp* w, Qcomp* r1, Qcomp* r2);
 /* nsteps IFRK2 steps (friction.cu:212-260 with TimeIntegration/IFRK2.cuh:16-51,
- * IFh := exp(L dt)), fused into six kernels per step. */
+ * IFh := exp(L dt)), fused into four kernel launches per step (row pass and
+ * column stage, twice) plus one column prologue after set_state. */
 int qs_step_ifrk2(qs_solver* s, int nsteps);
 /* Same step built from the per-op entry points above (reference call
  * granularity); for parity and for timing the unfused structure. */

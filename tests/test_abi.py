@@ -46,6 +46,23 @@ def test_struct_layouts_match_header(tmp_path):
     assert got == want
 
 
+def test_cpp_wrapper_header_builds_and_links(tmp_path):
+    """include/qsolver.hpp (reference-named C++ wrappers) compiles against the
+    C header and links against libqsolver.so (nothing is run)."""
+    src = tmp_path / "port.cpp"
+    src.write_text(
+        '#include "qsolver.hpp"\n'
+        "int main(){ Mesh* m = qsolver::make_mesh(64, 64, 6.283185307179586, 6.283185307179586);\n"
+        " Field* f = qsolver::make_field(m); qsolver::FwdTrans(f->spec, f->phys, m);\n"
+        " qsolver::IFRK2::integrate_func0D(f->spec, f->spec, f->spec, m->k_squared, m->k_squared, 1e-3, m);\n"
+        " qs_params p{}; p.lambda = p.Re = p.Er = p.alpha = p.D_Q = p.a_Q = p.c_Q = 1; p.dt = 1e-3;\n"
+        " qsolver::Solver s(m, p); s.step_ifrk2(1); qs_field_destroy(f); qs_mesh_destroy(m); return 0; }\n")
+    lib = os.path.dirname(Q.load()._name)
+    subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                           str(src), "-o", str(tmp_path / "port"), "-L", lib, "-lqsolver",
+                           "-Wl,-rpath," + lib, "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"])
+
+
 def test_synth_uniform_matches_splitmix64():
     got = Q.synth_uniform(12345, 0.0, 1.0, 1000)
     assert np.array_equal(got, splitmix64_uniform(12345, 1000))
