@@ -63,6 +63,14 @@ def lib():
             "or_solver_step": (None, [_P, _I, _I]),
             "or_solver_rhs": (None, [_P] * 7),
             "or_solver_lin": (None, [_P] * 7),
+            "or_adj_B": (None, [_P] * 3),
+            "or_adj_BT": (None, [_P] * 3),
+            "or_adj_F": (None, [_P] * 3),
+            "or_adj_FT": (None, [_P] * 3),
+            "or_adj_rhs_vjp": (None, [_P] * 11),
+            "or_adj_step_vjp": (None, [_P] * 11),
+            "or_adj_loss_grad": (_D, [_P, _P, _I, _P]),
+            "or_adj_gradient": (_D, [_P, _I, _I, _P, _P, _P]),
             "or_set_threads": (None, [_I]),
             "or_get_threads": (_I, []),
         }
@@ -202,3 +210,71 @@ class Solver:
         out = [np.empty(self.mesh.spec_shape) for _ in range(6)]
         lib().or_solver_lin(self.h, *(_ptr(a) for a in out))
         return out
+
+
+class Adjoint:
+    """Discrete adjoint of the oracle's IFRK2 step (exact transposes in the
+    real inner products of the stored arrays).  The control is the solver's
+    activity field; loss kind 0 = 1/2 mean(w^2), 1 = 1/2 mean(u^2 + v^2)."""
+
+    ENSTROPHY, KINETIC = 0, 1
+
+    def __init__(self, solver):
+        self.s = solver
+        self.m = solver.mesh
+
+    def _c(self, a):
+        return np.ascontiguousarray(a, dtype=np.complex128)
+
+    def _r(self, a):
+        return np.ascontiguousarray(a, dtype=np.float64)
+
+    def B(self, G):
+        f = np.empty(self.m.phys_shape)
+        lib().or_adj_B(self.s.h, _ptr(self._c(G)), _ptr(f))
+        return f
+
+    def BT(self, f):
+        G = np.empty(self.m.spec_shape, np.complex128)
+        lib().or_adj_BT(self.s.h, _ptr(self._r(f)), _ptr(G))
+        return G
+
+    def F(self, f):
+        G = np.empty(self.m.spec_shape, np.complex128)
+        lib().or_adj_F(self.s.h, _ptr(self._r(f)), _ptr(G))
+        return G
+
+    def FT(self, G):
+        f = np.empty(self.m.phys_shape)
+        lib().or_adj_FT(self.s.h, _ptr(self._c(G)), _ptr(f))
+        return f
+
+    def rhs_vjp(self, X, Nbar):
+        """Returns (Xbar (3 spectra), abar (physical)) = VJP of the right-hand side."""
+        ins = [self._c(a) for a in list(X) + list(Nbar)]
+        outs = [np.empty(self.m.spec_shape, np.complex128) for _ in range(3)]
+        abar = np.zeros(self.m.phys_shape)
+        lib().or_adj_rhs_vjp(self.s.h, *(_ptr(a) for a in ins + outs), _ptr(abar))
+        return outs, abar
+
+    def step_vjp(self, u, gbar):
+        ins = [self._c(a) for a in list(u) + list(gbar)]
+        outs = [np.empty(self.m.spec_shape, np.complex128) for _ in range(3)]
+        abar = np.zeros(self.m.phys_shape)
+        lib().or_adj_step_vjp(self.s.h, *(_ptr(a) for a in ins + outs), _ptr(abar))
+        return outs, abar
+
+    def loss_grad(self, W, kind):
+        Wb = np.empty(self.m.spec_shape, np.complex128)
+        L = lib().or_adj_loss_grad(self.s.h, _ptr(self._c(W)), int(kind), _ptr(Wb))
+        return L, Wb
+
+    def gradient(self, nsteps, kind, alpha_t=None):
+        """Loss after nsteps from the solver's state, dL/d(activity field) and,
+        with a per-step uniform activity alpha_t, dL/dalpha_t.  Leaves the
+        solver at the final state."""
+        ab = np.empty(self.m.phys_shape)
+        at = None if alpha_t is None else self._r(alpha_t)
+        abt = None if alpha_t is None else np.empty(nsteps)
+        L = lib().or_adj_gradient(self.s.h, int(nsteps), int(kind), _ptr(at), _ptr(ab), _ptr(abt))
+        return L, ab, abt

@@ -693,6 +693,255 @@ struct OSolver {
   }
 };
 
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the IFRK2 step (no reference counterpart: the reference
+// has no adjoint).  Exact transpose, in the real inner products
+//   <F, G> = sum_{j, i < Nxh} Re(F conj G)   (half spectra as stored)
+//   <f, g> = sum_{y, x} f g                   (physical fields),
+// of the right-hand side the functions above compute.  That right-hand side
+// equals, for every state X = (W, R1, R2) (BwdTrans's column-0 side effect
+// changes nothing downstream of a C2R), the compact form
+//   w = B W, uyvx = B((kx^2-ky^2) psi), u = B(i ky psi), v = B(-i kx psi),
+//   ux = B(-kx ky psi), r_a = B R_a, L_a = B(-k^2 R_a),  psi = W/k^2 (0 at k=0)
+//   N_w  = Re4[i kx' F(-u w) + i ky F(-v w)] + s(-kx^2 F P12 + 2 kx ky F P11 + ky^2 F P21)
+//   N_ra = F(loc_a) + i kx F(-u r_a) + i ky F(-v r_a)
+// with B the A10 C2R / (Nx Ny), F the unnormalised R2C, kx' = kx except 0 on
+// the Nyquist column (the x-derivative that BwdTrans/FwdTrans projects away,
+// friction/Qsolver.cuh:179-193), Re4 the real part at the four self-conjugate
+// bins, s = 1/(Er Re), and the pointwise terms of Qsolver.cuh:63-300.
+// Transposes used:  F^T G = NN B(Wt G) (Wt = 1 on columns 0 and Nx/2, 1/2
+// elsewhere);  B^T f = c F(f) / NN (c = 1 on those columns, 2 elsewhere);
+// multiplication by a complex c transposes to multiplication by conj(c).
+// ---------------------------------------------------------------------------
+struct OAdj {
+  OSolver* s;
+  OMesh* m;
+  std::vector<cd> tmp;
+  explicit OAdj(OSolver* sv) : s(sv), m(sv->mesh), tmp(sv->mesh->ns()) {}
+  long ns() const { return (long)m->ns(); }
+  long np() const { return (long)m->np(); }
+  double NN() const { return (double)m->Nx * m->Ny; }
+  bool edge(long index) const { const int i = (int)(index % m->Nxh); return i == 0 || 2 * i == m->Nx; }
+
+  void B(const cd* G, double* f) {   // A10 C2R / NN, input preserved
+    std::memcpy(tmp.data(), G, sizeof(cd) * ns());
+    c2r2d(m, tmp.data(), f);
+    coeff(f, m->Nx, m->Ny);
+  }
+  void F(const double* f, cd* G) {   // unnormalised R2C, input preserved
+    std::memcpy(m->phys.data(), f, sizeof(double) * np());
+    r2c2d(m, m->phys.data(), G);
+  }
+  void FT(const cd* G, double* f) {  // F^T
+    std::vector<cd> g(G, G + ns());
+    for (long k = 0; k < ns(); ++k) if (!edge(k)) g[k] = 0.5 * g[k];
+    B(g.data(), f);
+    const double nn = NN();
+    for (long k = 0; k < np(); ++k) f[k] *= nn;
+  }
+  void BT(const double* f, cd* G) {  // B^T
+    F(f, G);
+    const double nn = NN();
+    for (long k = 0; k < ns(); ++k) G[k] = (edge(k) ? 1.0 : 2.0) / nn * G[k];
+  }
+
+  struct Fields { std::vector<double> w, uyvx, u, v, ux, r1, r2, L1, L2; };
+  void forward_fields(const cd* W, const cd* R1, const cd* R2, Fields& f) {
+    const long n = ns();
+    const double *kx = m->kx.data(), *ky = m->ky.data(), *k2 = m->k_squared.data();
+    for (auto* v : {&f.w, &f.uyvx, &f.u, &f.v, &f.ux, &f.r1, &f.r2, &f.L1, &f.L2}) v->assign(np(), 0.0);
+    std::vector<cd> g(n);
+    auto from = [&](std::vector<double>& out, auto coef) {
+      for (long k = 0; k < n; ++k) g[k] = coef(k);
+      B(g.data(), out.data());
+    };
+    auto psi = [&](long k) { return k == 0 ? mk(0, 0) : W[k] / k2[k]; };
+    from(f.w, [&](long k) { return W[k]; });
+    from(f.uyvx, [&](long k) { return (kx[k] * kx[k] - ky[k] * ky[k]) * psi(k); });
+    from(f.u, [&](long k) { return im() * ky[k] * psi(k); });
+    from(f.v, [&](long k) { return -1.0 * kx[k] * im() * psi(k); });
+    from(f.ux, [&](long k) { return -kx[k] * ky[k] * psi(k); });
+    from(f.r1, [&](long k) { return R1[k]; });
+    from(f.r2, [&](long k) { return R2[k]; });
+    from(f.L1, [&](long k) { return -k2[k] * R1[k]; });
+    from(f.L2, [&](long k) { return -k2[k] * R2[k]; });
+  }
+
+  // (Xbar) += VJP of the right-hand side at X applied to Nbar; abar += the
+  // activity field's part (alpha enters P11, P12, P21 as -alpha r).
+  void rhs_vjp(const cd* W, const cd* R1, const cd* R2, const double* Ra, const cd* Nbw, const cd* Nbr1,
+               const cd* Nbr2, cd* Wb, cd* R1b, cd* R2b, double* abar) {
+    const long n = ns(), P = np();
+    const int Nx = m->Nx, Ny = m->Ny, Nxh = m->Nxh;
+    const double *kx = m->kx.data(), *ky = m->ky.data(), *k2 = m->k_squared.data();
+    const double sE = 1.0 / (s->Er * s->Re), lam = s->lambda, DQ = s->c.DQ, aQ = s->c.aQ, cQ = s->c.cQ;
+    Fields f;
+    forward_fields(W, R1, R2, f);
+    // spectral adjoints of the products
+    std::vector<cd> Ob(Nbw, Nbw + n);
+    for (int j : {0, Ny / 2})
+      for (int i : {0, Nx / 2}) Ob[(size_t)j * Nxh + i].y = 0.0;
+    enum { A_, B_, P11, P12, P21, L1c, L2c, U1, V1, U2, V2, NPROD };
+    std::vector<std::vector<double>> pb(NPROD, std::vector<double>(P));
+    std::vector<cd> g(n);
+    auto adj = [&](int which, auto coef) {
+      for (long k = 0; k < n; ++k) g[k] = coef(k);
+      FT(g.data(), pb[which].data());
+    };
+    auto ikx_nyq0 = [&](long k) { return (2 * (k % Nxh) == Nx) ? 0.0 : kx[k]; };
+    if (s->advect_w) {
+      adj(A_, [&](long k) { return mk(0.0, -ikx_nyq0(k)) * Ob[k]; });
+      adj(B_, [&](long k) { return mk(0.0, -ky[k]) * Ob[k]; });
+    }
+    adj(P12, [&](long k) { return (-sE * kx[k] * kx[k]) * Ob[k]; });
+    adj(P11, [&](long k) { return (2.0 * sE * kx[k] * ky[k]) * Ob[k]; });
+    adj(P21, [&](long k) { return (sE * ky[k] * ky[k]) * Ob[k]; });
+    adj(L1c, [&](long k) { return Nbr1[k]; });
+    adj(U1, [&](long k) { return mk(0.0, -kx[k]) * Nbr1[k]; });
+    adj(V1, [&](long k) { return mk(0.0, -ky[k]) * Nbr1[k]; });
+    adj(L2c, [&](long k) { return Nbr2[k]; });
+    adj(U2, [&](long k) { return mk(0.0, -kx[k]) * Nbr2[k]; });
+    adj(V2, [&](long k) { return mk(0.0, -ky[k]) * Nbr2[k]; });
+    // pointwise adjoint
+    Fields b;
+    for (auto* v : {&b.w, &b.uyvx, &b.u, &b.v, &b.ux, &b.r1, &b.r2, &b.L1, &b.L2}) v->assign(P, 0.0);
+    for (long x = 0; x < P; ++x) {
+      const double w = f.w[x], uyvx = f.uyvx[x], u = f.u[x], v = f.v[x], ux = f.ux[x];
+      const double r1 = f.r1[x], r2 = f.r2[x], L1 = f.L1[x], L2 = f.L2[x];
+      const double al = Ra[x];
+      const double rr = std::sqrt(r1 * r1 + r2 * r2), S = 2 * rr, cS2 = cQ * S * S;
+      const double gg = lam * S * (cS2 - aQ);
+      double wb = 0, uyvxb = 0, ub = 0, vb = 0, uxb = 0, r1b = 0, r2b = 0, L1b = 0, L2b = 0;
+      double Sb = 0, cS2b = 0, ggb = 0;
+      if (s->advect_w) {
+        const double Ab = pb[A_][x], Bb = pb[B_][x];
+        ub += -w * Ab; wb += -u * Ab;
+        vb += -w * Bb; wb += -v * Bb;
+      }
+      { const double t = pb[U1][x]; ub += -r1 * t; r1b += -u * t; }
+      { const double t = pb[V1][x]; vb += -r1 * t; r1b += -v * t; }
+      { const double t = pb[U2][x]; ub += -r2 * t; r2b += -u * t; }
+      { const double t = pb[V2][x]; vb += -r2 * t; r2b += -v * t; }
+      const double p11b = pb[P11][x], p12b = pb[P12][x], p21b = pb[P21][x];
+      const double s1b = p11b, s2b = p12b + p21b, crb = p12b - p21b;
+      r1b += -al * p11b;
+      r2b += -al * (p12b + p21b);
+      abar[x] += -r1 * p11b - r2 * (p12b + p21b);
+      { const double t = pb[L1c][x];   // loc1 = -w r2 - cS2 r1 + lam S ux
+        wb += -r2 * t; r2b += -w * t; cS2b += -r1 * t; r1b += -cS2 * t; Sb += lam * ux * t; uxb += lam * S * t; }
+      { const double t = pb[L2c][x];   // loc2 = w r1 - cS2 r2 + 1/2 lam S uyvx
+        wb += r1 * t; r1b += w * t; cS2b += -r2 * t; r2b += -cS2 * t; Sb += 0.5 * lam * uyvx * t;
+        uyvxb += 0.5 * lam * S * t; }
+      // cr = 2 DQ (L1 r2 - r1 L2)
+      L1b += 2 * DQ * r2 * crb; r2b += 2 * DQ * L1 * crb; r1b += -2 * DQ * L2 * crb; L2b += -2 * DQ * r1 * crb;
+      // s_a = -lam DQ L_a S + gg r_a
+      L1b += -lam * DQ * S * s1b; Sb += -lam * DQ * L1 * s1b; ggb += r1 * s1b; r1b += gg * s1b;
+      L2b += -lam * DQ * S * s2b; Sb += -lam * DQ * L2 * s2b; ggb += r2 * s2b; r2b += gg * s2b;
+      // gg = lam S (cS2 - aQ), cS2 = cQ S^2, S = 2 sqrt(r1^2 + r2^2)
+      Sb += ggb * lam * (cS2 - aQ);
+      cS2b += ggb * lam * S;
+      Sb += cS2b * 2.0 * cQ * S;
+      if (rr > 0) { r1b += Sb * 2.0 * r1 / rr; r2b += Sb * 2.0 * r2 / rr; }
+      b.w[x] = wb; b.uyvx[x] = uyvxb; b.u[x] = ub; b.v[x] = vb; b.ux[x] = uxb;
+      b.r1[x] = r1b; b.r2[x] = r2b; b.L1[x] = L1b; b.L2[x] = L2b;
+    }
+    // spectral adjoint of the inverse maps
+    std::vector<cd> t(n);
+    auto acc = [&](cd* out, const std::vector<double>& fb, auto coef) {
+      BT(fb.data(), t.data());
+      for (long k = 0; k < n; ++k) out[k] = out[k] + coef(k) * t[k];
+    };
+    auto ipsi = [&](long k) { return k == 0 ? 0.0 : 1.0 / k2[k]; };
+    acc(Wb, b.w, [&](long) { return mk(1, 0); });
+    acc(Wb, b.uyvx, [&](long k) { return mk((kx[k] * kx[k] - ky[k] * ky[k]) * ipsi(k), 0); });
+    acc(Wb, b.u, [&](long k) { return mk(0, -ky[k] * ipsi(k)); });
+    acc(Wb, b.v, [&](long k) { return mk(0, kx[k] * ipsi(k)); });
+    acc(Wb, b.ux, [&](long k) { return mk(-kx[k] * ky[k] * ipsi(k), 0); });
+    acc(R1b, b.r1, [&](long) { return mk(1, 0); });
+    acc(R1b, b.L1, [&](long k) { return mk(-k2[k], 0); });
+    acc(R2b, b.r2, [&](long) { return mk(1, 0); });
+    acc(R2b, b.L2, [&](long k) { return mk(-k2[k], 0); });
+  }
+
+  // One IFRK2 step's VJP: (u_n, gbar = dL/du_{n+1}) -> dL/du_n, abar += ...
+  // Forward (IFRK2.cuh:16-51, IFh = IF, then dealias):
+  //   c2 = E (u + dt N(u)),  u_{n+1} = M (E u + dt/2 E N(u) + dt/2 N(c2)).
+  void step_vjp(const cd* const* u, const cd* const* gbar, cd* const* ubar, double* abar) {
+    const long n = ns();
+    const double dt = s->dt;
+    const double* E[3] = {s->IFw.data(), s->IFr1.data(), s->IFr2.data()};
+    const double* Ra = s->Ra->phys.data();
+    std::vector<cd> N1[3], c2[3], lamv[3], nb[3], cb[3];
+    for (int a = 0; a < 3; ++a) { N1[a].resize(n); c2[a].resize(n); lamv[a].resize(n); nb[a].resize(n); cb[a].assign(n, mk(0, 0)); }
+    // forward recompute: N(u) through the oracle's own right-hand side
+    {
+      std::vector<cd> w(u[0], u[0] + n), r1(u[1], u[1] + n), r2(u[2], u[2] + n);
+      or_rhs_into(w.data(), r1.data(), r2.data(), N1[0].data(), N1[1].data(), N1[2].data());
+    }
+    for (int a = 0; a < 3; ++a)
+      for (long k = 0; k < n; ++k) c2[a][k] = (u[a][k] + dt * N1[a][k]) * E[a][k];
+    const double* cut = m->cutoff.data();
+    for (int a = 0; a < 3; ++a)
+      for (long k = 0; k < n; ++k) { lamv[a][k] = cut[k] * gbar[a][k]; nb[a][k] = 0.5 * dt * lamv[a][k]; }
+    rhs_vjp(c2[0].data(), c2[1].data(), c2[2].data(), Ra, nb[0].data(), nb[1].data(), nb[2].data(),
+            cb[0].data(), cb[1].data(), cb[2].data(), abar);
+    for (int a = 0; a < 3; ++a)
+      for (long k = 0; k < n; ++k) {
+        ubar[a][k] = E[a][k] * (lamv[a][k] + cb[a][k]);
+        nb[a][k] = E[a][k] * (0.5 * dt * lamv[a][k] + dt * cb[a][k]);
+        cb[a][k] = mk(0, 0);
+      }
+    rhs_vjp(u[0], u[1], u[2], Ra, nb[0].data(), nb[1].data(), nb[2].data(), cb[0].data(), cb[1].data(),
+            cb[2].data(), abar);
+    for (int a = 0; a < 3; ++a)
+      for (long k = 0; k < n; ++k) ubar[a][k] = ubar[a][k] + cb[a][k];
+  }
+
+  void or_rhs_into(cd* w, cd* r1, cd* r2, cd* wn, cd* r1n, cd* r2n) {
+    const size_t bb = sizeof(cd) * ns();
+    std::memcpy(s->wcurr->spec.data(), w, bb);
+    std::memcpy(s->r1curr->spec.data(), r1, bb);
+    std::memcpy(s->r2curr->spec.data(), r2, bb);
+    s->rhs();
+    std::memcpy(wn, s->wnonl->spec.data(), bb);
+    std::memcpy(r1n, s->r1nonl->spec.data(), bb);
+    std::memcpy(r2n, s->r2nonl->spec.data(), bb);
+  }
+
+  // Loss at a state and dL/dW (kind 0: 1/2 mean(w^2); kind 1: 1/2 mean(u^2 + v^2)).
+  double loss_grad(const cd* W, int kind, cd* Wb) {
+    const long n = ns(), P = np();
+    const double *kx = m->kx.data(), *ky = m->ky.data(), *k2 = m->k_squared.data();
+    const double nn = NN();
+    std::vector<cd> g(n), t(n);
+    std::vector<double> f(P), h(P);
+    double L = 0;
+    for (long k = 0; k < n; ++k) Wb[k] = mk(0, 0);
+    if (kind == 0) {
+      B(W, f.data());
+      for (long x = 0; x < P; ++x) { L += 0.5 * f[x] * f[x] / nn; f[x] /= nn; }
+      BT(f.data(), Wb);
+      return L;
+    }
+    for (int c = 0; c < 2; ++c) {
+      for (long k = 0; k < n; ++k) {
+        const double ip = k == 0 ? 0.0 : 1.0 / k2[k];
+        g[k] = c == 0 ? mk(0, ky[k] * ip) * W[k] : mk(0, -kx[k] * ip) * W[k];
+      }
+      B(g.data(), f.data());
+      for (long x = 0; x < P; ++x) { L += 0.5 * f[x] * f[x] / nn; f[x] /= nn; }
+      BT(f.data(), t.data());
+      for (long k = 0; k < n; ++k) {
+        const double ip = k == 0 ? 0.0 : 1.0 / k2[k];
+        const cd c2 = c == 0 ? mk(0, -ky[k] * ip) : mk(0, kx[k] * ip);
+        Wb[k] = Wb[k] + c2 * t[k];
+      }
+    }
+    return L;
+  }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -812,6 +1061,76 @@ void or_solver_lin(void* sv, double* IFw, double* IFwh, double* IFr1, double* IF
   std::memcpy(IFr1h, s->IFr1h.data(), b);
   std::memcpy(IFr2, s->IFr2.data(), b);
   std::memcpy(IFr2h, s->IFr2h.data(), b);
+}
+
+
+// ---- discrete adjoint (test infrastructure, no reference counterpart) ----
+void or_adj_B(void* sv, const or_comp* G, double* f) { OAdj(static_cast<OSolver*>(sv)).B(G, f); }
+void or_adj_BT(void* sv, const double* f, or_comp* G) { OAdj(static_cast<OSolver*>(sv)).BT(f, G); }
+void or_adj_F(void* sv, const double* f, or_comp* G) { OAdj(static_cast<OSolver*>(sv)).F(f, G); }
+void or_adj_FT(void* sv, const or_comp* G, double* f) { OAdj(static_cast<OSolver*>(sv)).FT(G, f); }
+void or_adj_rhs_vjp(void* sv, const or_comp* W, const or_comp* R1, const or_comp* R2, const or_comp* Nbw,
+                    const or_comp* Nbr1, const or_comp* Nbr2, or_comp* Wb, or_comp* R1b, or_comp* R2b,
+                    double* abar) {
+  OSolver* s = static_cast<OSolver*>(sv);
+  const size_t n = s->mesh->ns();
+  for (or_comp* p : {Wb, R1b, R2b}) std::memset(p, 0, sizeof(or_comp) * n);
+  OAdj(s).rhs_vjp(W, R1, R2, s->Ra->phys.data(), Nbw, Nbr1, Nbr2, Wb, R1b, R2b, abar);
+}
+void or_adj_step_vjp(void* sv, const or_comp* W, const or_comp* R1, const or_comp* R2, const or_comp* gW,
+                     const or_comp* gR1, const or_comp* gR2, or_comp* uW, or_comp* uR1, or_comp* uR2,
+                     double* abar) {
+  const or_comp* u[3] = {W, R1, R2};
+  const or_comp* g[3] = {gW, gR1, gR2};
+  or_comp* ub[3] = {uW, uR1, uR2};
+  OAdj(static_cast<OSolver*>(sv)).step_vjp(u, g, ub, abar);
+}
+double or_adj_loss_grad(void* sv, const or_comp* W, int kind, or_comp* Wb) {
+  return OAdj(static_cast<OSolver*>(sv)).loss_grad(W, kind, Wb);
+}
+// Gradient of the loss after nsteps IFRK2 steps from the solver's state with
+// respect to the activity: the field (abar_field, Nx*Ny, may be NULL) and, when
+// alpha_t is given (step n then uses the uniform activity alpha_t[n]), its
+// per-step sums (abar_t[n]).  Stores every state (small sizes only).  The
+// solver's state is left at the final step.  Returns the loss.
+double or_adj_gradient(void* sv, int nsteps, int kind, const double* alpha_t, double* abar_field,
+                       double* abar_t) {
+  OSolver* s = static_cast<OSolver*>(sv);
+  OMesh* m = s->mesh;
+  const size_t n = m->ns(), P = m->np();
+  OAdj A(s);
+  std::vector<std::vector<cd>> st(3 * (size_t)(nsteps + 1), std::vector<cd>(n));
+  auto save = [&](int k) {
+    std::memcpy(st[3 * k + 0].data(), s->wold->spec.data(), sizeof(cd) * n);
+    std::memcpy(st[3 * k + 1].data(), s->r1old->spec.data(), sizeof(cd) * n);
+    std::memcpy(st[3 * k + 2].data(), s->r2old->spec.data(), sizeof(cd) * n);
+  };
+  std::vector<double> Ra_keep(s->Ra->phys);
+  save(0);
+  for (int k = 0; k < nsteps; ++k) {
+    if (alpha_t) FldSet(s->Ra->phys.data(), alpha_t[k], (long)P);
+    s->step_ifrk2();
+    save(k + 1);
+  }
+  std::vector<cd> g[3], ub[3];
+  for (int a = 0; a < 3; ++a) { g[a].assign(n, mk(0, 0)); ub[a].assign(n, mk(0, 0)); }
+  const double L = A.loss_grad(st[3 * nsteps].data(), kind, g[0].data());
+  std::vector<double> ab(P, 0.0), abk(P);
+  for (int k = nsteps - 1; k >= 0; --k) {
+    if (alpha_t) FldSet(s->Ra->phys.data(), alpha_t[k], (long)P);
+    std::fill(abk.begin(), abk.end(), 0.0);
+    const cd* u[3] = {st[3 * k].data(), st[3 * k + 1].data(), st[3 * k + 2].data()};
+    const cd* gg[3] = {g[0].data(), g[1].data(), g[2].data()};
+    cd* uu[3] = {ub[0].data(), ub[1].data(), ub[2].data()};
+    A.step_vjp(u, gg, uu, abk.data());
+    for (int a = 0; a < 3; ++a) g[a].swap(ub[a]);
+    double sum = 0;
+    for (size_t x = 0; x < P; ++x) { ab[x] += abk[x]; sum += abk[x]; }
+    if (abar_t) abar_t[k] = sum;
+  }
+  if (abar_field) std::memcpy(abar_field, ab.data(), sizeof(double) * P);
+  s->Ra->phys = Ra_keep;
+  return L;
 }
 
 void or_set_threads(int n) { g_threads = n; }

@@ -90,6 +90,31 @@ void  or_solver_rhs(void* s, or_comp* w, or_comp* r1, or_comp* r2,
 void  or_solver_lin(void* s, double* IFw, double* IFwh, double* IFr1,
                     double* IFr1h, double* IFr2, double* IFr2h);
 
+/*
+ * Discrete adjoint of the IFRK2 step (no reference counterpart).  Exact
+ * transposes in the real inner products of the stored arrays:
+ *   B = A10 C2R / (Nx Ny), B^T;  F = unnormalised R2C, F^T;
+ *   rhs_vjp: (X, Nbar) -> Xbar = (dN/dX)^T Nbar, abar += (dN/dalpha)^T Nbar
+ *   step_vjp: (u_n, dL/du_{n+1}) -> dL/du_n, abar += ...
+ *   loss_grad: kind 0 = 1/2 mean(w^2), kind 1 = 1/2 mean(u^2 + v^2); returns L.
+ *   gradient: nsteps from the solver's state; alpha_t (nsteps, or NULL) gives
+ *   a uniform activity per step and abar_t its per-step gradient sums.
+ * The solver's activity field (or_solver_set_activity) is the control alpha(x).
+ */
+void   or_adj_B(void* s, const or_comp* G, double* f);
+void   or_adj_BT(void* s, const double* f, or_comp* G);
+void   or_adj_F(void* s, const double* f, or_comp* G);
+void   or_adj_FT(void* s, const or_comp* G, double* f);
+void   or_adj_rhs_vjp(void* s, const or_comp* W, const or_comp* R1, const or_comp* R2,
+                      const or_comp* Nbw, const or_comp* Nbr1, const or_comp* Nbr2,
+                      or_comp* Wb, or_comp* R1b, or_comp* R2b, double* abar);
+void   or_adj_step_vjp(void* s, const or_comp* W, const or_comp* R1, const or_comp* R2,
+                       const or_comp* gW, const or_comp* gR1, const or_comp* gR2,
+                       or_comp* uW, or_comp* uR1, or_comp* uR2, double* abar);
+double or_adj_loss_grad(void* s, const or_comp* W, int kind, or_comp* Wb);
+double or_adj_gradient(void* s, int nsteps, int kind, const double* alpha_t,
+                       double* abar_field, double* abar_t);
+
 /* Threads used by the oracle's OpenMP loops (0 = library default). */
 void  or_set_threads(int n);
 int   or_get_threads(void);
