@@ -31,35 +31,10 @@
 //  the projection the reference's BwdTrans/FwdTrans round trip applies to the
 //  advection term, friction/Qsolver.cuh:179-193), N_r1 = F3 + i ky F4,
 //  N_r2 = F5 + i ky F6.
-#include "qs_fused.hpp"
+#include "qs_fused_dev.hpp"
 
 namespace qs {
 namespace {
-
-constexpr int kBuf = 1088;           // LDS entries of one wave buffer (1024 + 1024/16)
-constexpr int kTwq = 256;            // quarter twiddle table of W_1024
-
-// W_N^m from the quarter table of W_1024 (N <= 1024, power of two).
-template <int N>
-struct TwN {
-  const double2* p;
-  __device__ __forceinline__ double2 operator()(int m) const {
-    return TwQuarter<1024>{p}(m * (1024 / N));
-  }
-};
-
-// ===========================================================================
-// Row job
-// ===========================================================================
-
-// Z_k of a two-for-one C2R (Hermitian extensions of X and Y; imaginary parts
-// of the DC and Nyquist bins discarded).
-template <int N>
-__device__ __forceinline__ double2 herm_pair(int k, double2 X, double2 Y) {
-  if (k == 0 || 2 * k == N) return make_double2(X.x, Y.x);
-  if (2 * k < N) return make_double2(X.x - Y.y, X.y + Y.x);
-  return make_double2(X.x + Y.y, Y.x - X.y);
-}
 
 // Row job layout (one job = NV = 1024/N rows, 6 waves, 6 LDS buffers):
 //  1. the 7 mixed rows are loaded cooperatively (coalesced) into a staging
@@ -83,84 +58,14 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;     // lane = lp * NV + g
-  const double sc = a.scale;
   double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
 
-  // ---- 1. stage the mixed rows (array t, row vec, bin k) at t*RH + vec*H + k --
-  // RH = 512 + NV <= 576 < 2 * 384: two slots per thread and array, all 14
-  // loads in flight before the first LDS write.
-  static_assert(RH <= 2 * 384, "row staging assumes two slots per thread");
-  {
-    double2 ld[7][2];
-    size_t off[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r = tid + 384 * h;
-      const int rr = r < RH ? r : RH - 1;
-      const int vec = rr / H, k = rr % H;
-      const int y = row0 + vec;
-      off[h] = (size_t)(y < a.Ny ? y : a.Ny - 1) * a.pitch + k;
-    }
-#pragma unroll
-    for (int t = 0; t < 7; ++t)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) ld[t][h] = gload(a.mix[t] + off[h]);
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int t = 0; t < 7; ++t)
-#pragma unroll
-      for (int h = 0; h < 2; ++h)
-        if (tid + 384 * h < RH) sm[t * RH + tid + 384 * h] = ld[t][h];
-  }
-  __syncthreads();
-
-  // ---- 2. C2R inputs --------------------------------------------------------
+  stage_rows<N>(a.minp, a, row0, sm);
   double2 v[16];
-  if (wave < 5) {
-    const double w00 = gload(a.w00);
-    const double2* st = sm + g * H;
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int k = lp + LPC * m;
-      const int bin = k > N / 2 ? N - k : k;
-      const double kx = a.dkx * bin;
-      double2 X, Y;
-      if (wave == 0) {            // (w, dy u + dx v = -W + 2 kx^2 psi + W00 [kx = 0])
-        const double2 W = st[0 * RH + bin], ps = st[1 * RH + bin];
-        const double t2 = 2.0 * kx * kx;
-        X = W;
-        Y = make_double2(-W.x + t2 * ps.x + (bin == 0 ? w00 : 0.0), -W.y + t2 * ps.y);
-      } else if (wave == 1) {     // (u = i ky psi, v = -i kx psi)
-        const double2 ps = st[1 * RH + bin];
-        X = st[2 * RH + bin];
-        Y = make_double2(kx * ps.y, -kx * ps.x);
-      } else if (wave == 2) {     // (r1, r2)
-        X = st[3 * RH + bin];
-        Y = st[5 * RH + bin];
-      } else if (wave == 3) {     // (L1, L2) = -kx^2 R + R_yy
-        const double2 R1 = st[3 * RH + bin], R1yy = st[4 * RH + bin];
-        const double2 R2 = st[5 * RH + bin], R2yy = st[6 * RH + bin];
-        X = make_double2(-kx * kx * R1.x + R1yy.x, -kx * kx * R1.y + R1yy.y);
-        Y = make_double2(-kx * kx * R2.x + R2yy.x, -kx * kx * R2.y + R2yy.y);
-      } else {                    // (dx u = i kx u, -)
-        const double2 U = st[2 * RH + bin];
-        X = make_double2(-kx * U.y, kx * U.x);
-        Y = make_double2(0.0, 0.0);
-      }
-      v[m] = herm_pair<N>(k, X, Y);
-    }
-  }
-  __syncthreads();   // staging area free
-
-  // ---- 3. inverse x FFTs -> physical fields ------------------------------------
-  if (wave < 5) {
-    wave_fft<N, +1>(v, vb, lp, tw);
-#pragma unroll
-    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = make_double2(v[m].x * sc, v[m].y * sc);
-  }
-  __syncthreads();
+  inverse_fields<N>(a, row0, sm, tw, v);
 
   // ---- 4. pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) --
+  const double alu = a.alpha_p ? gload(a.alpha_p) : a.alpha;   // uniform activity of this step
   for (int p = tid; p < 1024; p += 384) {
     const int gg = p / N, x = p % N;
     const int o = gg * VS + swz(x);
@@ -169,7 +74,7 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
                   f3 = sm[3 * kBuf + o], f4 = sm[4 * kBuf + o];
     const double w = f0.x, uyvx = f0.y, u = f1.x, v = f1.y, r1 = f2.x, r2 = f2.y;
     const double L1 = f3.x, L2 = f3.y, ux = f4.x;
-    const double al = a.Ra ? gload(a.Ra + (size_t)(yy < a.Ny ? yy : a.Ny - 1) * N + x) : a.alpha;
+    const double al = a.Ra ? gload(a.Ra + (size_t)(yy < a.Ny ? yy : a.Ny - 1) * N + x) : alu;
     const double S = 2 * sqrt(r1 * r1 + r2 * r2);
     const double cS2 = a.cQ * S * S;
     const double gg1 = a.lambda * S * (cS2 - a.aQ);
@@ -236,79 +141,27 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
 // ===========================================================================
 // Column job
 // ===========================================================================
-// Tile of TC = 4 * 1024/NY columns: wave w owns columns i0 + w*CW + g (g <
-// CW = 1024/NY).  Global traffic is staged through the wave buffers so that
-// every load / store instruction covers whole row segments of the tile
-// (coalesced), while the FFTs run per wavefront on the lane layout
-// "lane (lp, g) holds rows lp + LPC*m".
+// Column job: one field (FLD 0 = w, 1 = r1, 2 = r2) on a tile of 4*1024/NY
+// columns (ColTile), one column per lane group, FFTs along y per wavefront.
 template <int NY, int MODE>
 __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq) {
-  constexpr int CW = 1024 / NY;          // columns per wave
-  constexpr int TC = 4 * CW;             // columns per tile
   constexpr int LPC = NY / 16;
-  constexpr int VS = NY + NY / 16;
-  constexpr int WS = kBuf + 1;           // wave buffer stride (odd: spreads LDS banks)
   const TwN<NY> tw{twq};
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int lp = lane / CW, g = lane % CW;     // lane = lp * CW + g
-  const int i = i0 + wave * CW + g;
+  ColTile<NY> T(a, sm, i0);
+  const int lp = T.lp;
+  const int i = i0 + (threadIdx.x >> 6) * ColTile<NY>::CW + T.g;
   const bool iok = i < a.Nxh;
   const int ic = iok ? i : 0;
-  double2* vb = sm + wave * WS + g * VS;
+  double2* vb = T.vb;
   const double kx = a.dkx * ic;
   // ky(j) of the mesh tables (Basic/Mesh.cuh:6-21)
   auto kyj = [&](int j) { return a.dky * (j < NY / 2 + 1 ? j : j - NY); };
   const int nyq_x = a.Nx / 2;
-  const size_t pitch = a.pitch;
   bool bad = false;
-
-  // tile element (row j, column c) <-> LDS slot
-  auto slot = [&](int j, int c) { return (c / CW) * WS + (c % CW) * VS + swz(j); };
-  // cooperative coalesced tile load into the wave buffers (ends with a barrier)
-  auto tile_load = [&](const double2* __restrict__ src) {
-    // all 16 loads in flight before the first LDS write (the scheduling
-    // barrier keeps the compiler from pairing each load with its write)
-    double2 ld[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int e = tid + 256 * q;
-      const int c = e % TC, j = e / TC;
-      const int ii = i0 + c;
-      ld[q] = gload(src + (size_t)j * pitch + (ii < a.Nxh ? ii : a.Nxh - 1));
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int e = tid + 256 * q;
-      const int c = e % TC, j = e / TC;
-      sm[slot(j, c)] = i0 + c < a.Nxh ? ld[q] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-  };
-  // cooperative coalesced tile store from the wave buffers
-  auto tile_store = [&](double2* __restrict__ dst) {
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int e = tid + 256 * q;
-      const int c = e % TC, j = e / TC;
-      const int ii = i0 + c;
-      if (ii < a.Nxh) gstore(dst + (size_t)j * pitch + ii, sm[slot(j, c)]);
-    }
-  };
-  auto stage_in = [&](const double2* __restrict__ src, double2 (&v)[16]) {
-    tile_load(src);
-#pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = vb[swz(lp + LPC * m)];
-    wave_sync();
-  };
-  auto stage_out = [&](double2* __restrict__ dst, const double2 (&v)[16]) {
-#pragma unroll
-    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = v[m];
-    tile_store(dst);
-  };
+  auto stage_in = [&](const double2* __restrict__ src, double2 (&v)[16]) { T.stage_in(src, v); };
+  auto stage_out = [&](double2* __restrict__ dst, const double2 (&v)[16]) { T.stage_out(dst, v); };
+  auto tile_load = [&](const double2* __restrict__ src) { T.load(src); };
+  auto tile_store = [&](double2* __restrict__ dst) { T.store(dst); };
 
   double2 st[16];   // the state of the next stage (curr / new / old)
   if (MODE == MODE_PRO) {
@@ -376,13 +229,13 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   // ---- inverse set of the next stage from st ----------------------------------
   if (FLD == 0) {
     // M0 = W, M1 = psi = W/k^2, M2 = i ky psi   (vel_func, QFldfuncs.cuh:204-224)
-    if (i == 0 && lp == 0 && a.w00) *(QS_GLOBAL double*)a.w00 = st[0].x;   // element (j=0, i=0)
+    if (i == 0 && lp == 0 && a.w00o) *(QS_GLOBAL double*)a.w00o = st[0].x;   // element (j=0, i=0)
     {
       double2 v[16];
 #pragma unroll
       for (int m = 0; m < 16; ++m) v[m] = st[m];
       wave_fft<NY, +1>(v, vb, lp, tw);
-      stage_out(a.mix[0], v);
+      stage_out(a.mout[0], v);
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) {   // st <- psi
@@ -396,7 +249,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
 #pragma unroll
       for (int m = 0; m < 16; ++m) v[m] = st[m];
       wave_fft<NY, +1>(v, vb, lp, tw);
-      stage_out(a.mix[1], v);
+      stage_out(a.mout[1], v);
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) {   // st <- i ky psi
@@ -404,7 +257,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       st[m] = make_double2(-ky * st[m].y, ky * st[m].x);
     }
     wave_fft<NY, +1>(st, vb, lp, tw);
-    stage_out(a.mix[2], st);
+    stage_out(a.mout[2], st);
   } else {
     const int m0 = FLD == 1 ? 3 : 5;
     {
@@ -412,7 +265,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
 #pragma unroll
       for (int m = 0; m < 16; ++m) v[m] = st[m];
       wave_fft<NY, +1>(v, vb, lp, tw);
-      stage_out(a.mix[m0], v);
+      stage_out(a.mout[m0], v);
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) {   // st <- -ky^2 R
@@ -420,7 +273,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       st[m] = make_double2(-ky * ky * st[m].x, -ky * ky * st[m].y);
     }
     wave_fft<NY, +1>(st, vb, lp, tw);
-    stage_out(a.mix[m0 + 1], st);
+    stage_out(a.mout[m0 + 1], st);
   }
   if (MODE == MODE_S2 && bad) atomicOr(a.flag, 1);
 }
@@ -429,9 +282,6 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
 // ===========================================================================
 // Kernels: one run (grid over jobs) and an ensemble (work list over runs).
 // ===========================================================================
-__device__ __forceinline__ void stage_twiddles(double2* twq, const double2* tw1024, int tid, int nt) {
-  for (int m = tid; m < kTwq; m += nt) twq[m] = tw1024[m];
-}
 
 template <int N>
 __device__ __forceinline__ void row_dispatch_n(const RunDesc& a, int job, double2* sm, const double2* twq) {

@@ -6,19 +6,34 @@
 namespace qs {
 
 enum { MODE_PRO = 0, MODE_S1 = 1, MODE_S2 = 2 };
+// Column kernels of the adjoint step (see qs_adjoint.hip).
+enum { AMODE_START = 0, AMODE_B = 1, AMODE_A = 2, AMODE_END = 3 };
 
 // Everything a fused job needs about one run (device-resident array).
 struct RunDesc {
   int Nx, Ny, Nxh, pitch;         // pitch: row stride (complex) of state and mixed arrays
-  double2* mix[7];                // mixed arrays [Ny][pitch]
+  double2* mix[7];                // forward set [Ny][pitch] (row pass writes, column reads);
+                                  // adjoint: Fbar (column writes) then Mbar (row adjoint, in place)
+  double2* minp[7];               // inverse set read by the row pass
+  double2* mout[7];               // inverse set written by the column kernels
   double2* old_[3];               // state w, r1, r2 [Ny][pitch]
   double2* new_[3];               // stage-1 partial sums
-  const double* Ra;               // activity field [Ny][Nx] or null (uniform alpha)
-  double* w00;                    // W(0,0) of the current stage
+  const double* Ra;               // activity field [Ny][Nx] or null (uniform activity)
+  const double* alpha_p;          // uniform activity of this step (device scalar) or null (alpha)
+  double* w00;                    // W(0,0) of the stage the row pass reads
+  double* w00o;                   // W(0,0) of the stage the column kernel writes
   int* flag;                      // non-finite flag
   double dkx, dky;                // 2 pi / Lx, 2 pi / Ly: kx(i) = dkx i, ky(j) = dky (j or j - Ny)
   double alpha, lambda, DQ, aQ, cQ, s_ErRe, scale, dt, Re, Rf;
   int advect_w;
+  // ---- discrete adjoint -------------------------------------------------
+  double2* lam_[3];               // masked dL/du_{n+1} (lambda) / incoming gradient
+  double2* acc_[3];               // E (lambda + cbar2): the stage-2 part of dL/du_n
+  double2* uout[3];               // dL/du_0 [Ny][pitch] or null
+  double* abar;                   // dL/dRa [Ny][Nx] (accumulated) or null
+  double* arow;                   // per-row sums of dL/dalpha of this step [Ny] (accumulated) or null
+  double* w00bar;                 // per-row parts of dL/dRe W(0,0) [Ny] (row adjoint -> column)
+  double2* scratch;               // row adjoint: [grid][6][1024] physical product adjoints
 };
 
 // Jobs of one run per launch (row: 1024/Nx rows each; column: 3 fields x tiles).
@@ -32,6 +47,12 @@ int launch_fused_row(int nfix, const RunDesc* runs, const int2* work, int nwork,
                      const double2* tw1024, hipStream_t st);
 int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, int nwork, int grid,
                      const double2* tw1024, hipStream_t st);
+
+// Discrete adjoint kernels (qs_adjoint.hip).  nfix > 0 only.
+int launch_adj_row(int nfix, const RunDesc* runs, const int2* work, int nwork, int grid,
+                   const double2* tw1024, hipStream_t st);
+int launch_adj_col(int nfix, int amode, const RunDesc* runs, const int2* work, int nwork, int grid,
+                   const double2* tw1024, hipStream_t st);
 
 int wave_fft_test(int n, int sign, const double2* in, double2* out, const double2* tw1024, hipStream_t st);
 

@@ -1,0 +1,200 @@
+// qs_fused_dev.hpp -- device building blocks shared by the fused IFRK2 kernels
+// (qs_fused.hip) and their discrete adjoint (qs_adjoint.hip).
+#pragma once
+
+#include "qs_fused.hpp"
+
+namespace qs {
+namespace {
+
+constexpr int kBuf = 1088;           // LDS entries of one wave buffer (1024 + 1024/16)
+constexpr int kTwq = 256;            // quarter twiddle table of W_1024
+
+// W_N^m from the quarter table of W_1024 (N <= 1024, power of two).
+template <int N>
+struct TwN {
+  const double2* p;
+  __device__ __forceinline__ double2 operator()(int m) const {
+    return TwQuarter<1024>{p}(m * (1024 / N));
+  }
+};
+
+// ===========================================================================
+// Row job
+// ===========================================================================
+
+// Z_k of a two-for-one C2R (Hermitian extensions of X and Y; imaginary parts
+// of the DC and Nyquist bins discarded).
+template <int N>
+__device__ __forceinline__ double2 herm_pair(int k, double2 X, double2 Y) {
+  if (k == 0 || 2 * k == N) return make_double2(X.x, Y.x);
+  if (2 * k < N) return make_double2(X.x - Y.y, X.y + Y.x);
+  return make_double2(X.x + Y.y, Y.x - X.y);
+}
+
+__device__ __forceinline__ void stage_twiddles(double2* twq, const double2* tw1024, int tid, int nt) {
+  for (int m = tid; m < kTwq; m += nt) twq[m] = tw1024[m];
+}
+
+// Row staging: rows row0 .. row0+NV-1 of the 7 arrays arr[0..6] (x-spectral,
+// H = N/2+1 bins each) into sm[t*RH + vec*H + k].  RH = 512 + NV <= 576 < 2*384:
+// two slots per thread and array, all 14 loads in flight before the first LDS
+// write.  Ends with a barrier.
+template <int N>
+__device__ __forceinline__ void stage_rows(double2* const* arr, const RunDesc& a, int row0, double2* sm) {
+  constexpr int NV = 1024 / N, H = N / 2 + 1, RH = NV * H;
+  static_assert(RH <= 2 * 384, "row staging assumes two slots per thread");
+  const int tid = threadIdx.x;
+  double2 ld[7][2];
+  unsigned off[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int r = tid + 384 * h;
+    const int rr = r < RH ? r : RH - 1;
+    const int vec = rr / H, k = rr % H;
+    const int y = row0 + vec;
+    off[h] = (unsigned)((y < a.Ny ? y : a.Ny - 1) * a.pitch + k);
+  }
+#pragma unroll
+  for (int t = 0; t < 7; ++t)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) ld[t][h] = gload(arr[t] + off[h]);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int t = 0; t < 7; ++t)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if (tid + 384 * h < RH) sm[t * RH + tid + 384 * h] = ld[t][h];
+  __syncthreads();
+}
+
+// Physical fields of a row job from the staged inverse set (after stage_rows
+// of the inverse-set arrays).  Waves 0-4 build two-for-one C2R inputs, pairing
+// fields of like magnitude, and run the inverse x FFTs; the fields, scaled by
+// 1/(Nx Ny), land in wave buffers 0-4 as pairs
+//   0: (w, dy u + dx v)   1: (u, v)   2: (r1, r2)   3: (L1, L2)   4: (dx u, 0)
+// (vector g of wave buffer q at sm[q*kBuf + g*VS + swz(x)]).  v: scratch
+// registers of the caller.  Ends with a barrier.
+template <int N, class TW>
+__device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, double2* sm, const TW& tw,
+                                               double2 (&v)[16]) {
+  constexpr int NV = 1024 / N, LPC = N / 16, VS = N + N / 16, H = N / 2 + 1, RH = NV * H;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int lp = lane / NV, g = lane % NV;
+  double2* vb = sm + wave * kBuf + g * VS;
+  if (wave < 5) {
+    const double w00 = gload(a.w00);
+    const double2* st = sm + g * H;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int k = lp + LPC * m;
+      const int bin = k > N / 2 ? N - k : k;
+      const double kx = a.dkx * bin;
+      double2 X, Y;
+      if (wave == 0) {            // (w, dy u + dx v = -W + 2 kx^2 psi + W00 [kx = 0])
+        const double2 W = st[0 * RH + bin], ps = st[1 * RH + bin];
+        const double t2 = 2.0 * kx * kx;
+        X = W;
+        Y = make_double2(-W.x + t2 * ps.x + (bin == 0 ? w00 : 0.0), -W.y + t2 * ps.y);
+      } else if (wave == 1) {     // (u = i ky psi, v = -i kx psi)
+        const double2 ps = st[1 * RH + bin];
+        X = st[2 * RH + bin];
+        Y = make_double2(kx * ps.y, -kx * ps.x);
+      } else if (wave == 2) {     // (r1, r2)
+        X = st[3 * RH + bin];
+        Y = st[5 * RH + bin];
+      } else if (wave == 3) {     // (L1, L2) = -kx^2 R + R_yy
+        const double2 R1 = st[3 * RH + bin], R1yy = st[4 * RH + bin];
+        const double2 R2 = st[5 * RH + bin], R2yy = st[6 * RH + bin];
+        X = make_double2(-kx * kx * R1.x + R1yy.x, -kx * kx * R1.y + R1yy.y);
+        Y = make_double2(-kx * kx * R2.x + R2yy.x, -kx * kx * R2.y + R2yy.y);
+      } else {                    // (dx u = i kx u, -)
+        const double2 U = st[2 * RH + bin];
+        X = make_double2(-kx * U.y, kx * U.x);
+        Y = make_double2(0.0, 0.0);
+      }
+      v[m] = herm_pair<N>(k, X, Y);
+    }
+  }
+  __syncthreads();   // staging area free
+  if (wave < 5) {
+    const double sc = a.scale;
+    wave_fft<N, +1>(v, vb, lp, tw);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = make_double2(v[m].x * sc, v[m].y * sc);
+  }
+  __syncthreads();
+}
+
+// Column tile of TC = 4 * 1024/NY columns starting at i0: wave w owns columns
+// i0 + w*CW + g (g < CW = 1024/NY).  Global traffic is staged through the wave
+// buffers (stride WS) so every load / store instruction covers whole row
+// segments of the tile, while the FFTs run per wavefront on the lane layout
+// "lane (lp, g) holds rows lp + LPC*m".
+template <int NY>
+struct ColTile {
+  static constexpr int CW = 1024 / NY, TC = 4 * CW, LPC = NY / 16, VS = NY + NY / 16;
+  static constexpr int WS = kBuf + 1;   // odd wave stride: spreads LDS banks
+  const RunDesc& a;
+  double2* sm;
+  int i0, tid, lp, g;
+  double2* vb;
+  __device__ __forceinline__ ColTile(const RunDesc& a_, double2* sm_, int i0_) : a(a_), sm(sm_), i0(i0_) {
+    tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    lp = lane / CW;
+    g = lane % CW;
+    vb = sm + wave * WS + g * VS;
+  }
+  __device__ __forceinline__ int slot(int j, int c) const { return (c / CW) * WS + (c % CW) * VS + swz(j); }
+  // cooperative coalesced tile load into the wave buffers (ends with a barrier);
+  // all 16 loads are in flight before the first LDS write (the scheduling
+  // barrier keeps the compiler from pairing each load with its write)
+  __device__ __forceinline__ void load(const double2* __restrict__ src) {
+    double2 ld[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q;
+      const int c = e % TC, j = e / TC;
+      const int ii = i0 + c;
+      // 32-bit element offset: one VGPR per load (SGPR base + VGPR offset form)
+      const unsigned o = (unsigned)(j * a.pitch + (ii < a.Nxh ? ii : a.Nxh - 1));
+      ld[q] = gload(src + o);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q;
+      const int c = e % TC, j = e / TC;
+      sm[slot(j, c)] = i0 + c < a.Nxh ? ld[q] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+  }
+  // cooperative coalesced tile store from the wave buffers
+  __device__ __forceinline__ void store(double2* __restrict__ dst) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q;
+      const int c = e % TC, j = e / TC;
+      const int ii = i0 + c;
+      if (ii < a.Nxh) gstore(dst + (unsigned)(j * a.pitch + ii), sm[slot(j, c)]);
+    }
+  }
+  __device__ __forceinline__ void stage_in(const double2* __restrict__ src, double2 (&v)[16]) {
+    load(src);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = vb[swz(lp + LPC * m)];
+    wave_sync();
+  }
+  __device__ __forceinline__ void stage_out(double2* __restrict__ dst, const double2 (&v)[16]) {
+#pragma unroll
+    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = v[m];
+    store(dst);
+  }
+};
+
+}  // namespace
+}  // namespace qs

@@ -51,10 +51,10 @@ RunDesc make_desc(qs_solver* S) {
   const Mesh* m = S->mesh;
   RunDesc d{};
   d.Nx = m->Nx; d.Ny = m->Ny; d.Nxh = m->Nxh; d.pitch = S->pitch;
-  for (int k = 0; k < 7; ++k) d.mix[k] = S->mix[k];
+  for (int k = 0; k < 7; ++k) d.mix[k] = d.minp[k] = d.mout[k] = S->mix[k];
   for (int k = 0; k < 3; ++k) { d.old_[k] = S->old_[k]; d.new_[k] = S->new_[k]; }
   d.Ra = S->Ra;
-  d.w00 = S->w00;
+  d.w00 = d.w00o = S->w00;
   d.flag = S->flag;
   d.dkx = 2 * M_PI / m->Lx;   // the mesh tables' own expressions (Basic/Mesh.cuh:6-21)
   d.dky = 2 * M_PI / m->Ly;
