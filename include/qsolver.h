@@ -214,6 +214,33 @@ int qs_solver_debug_array(qs_solver* s, int which, Qcomp* out);
  * sign -1 forward / +1 inverse, unnormalised) on 1024/n contiguous vectors. */
 int qs_debug_wave_fft(const Mesh* mesh, int n, int sign, const Qcomp* in, Qcomp* out);
 
+/* ---- discrete adjoint of the IFRK2 step ----------------------------------
+ * No reference counterpart (the reference has no adjoint).  Exact transpose of
+ * qs_step_ifrk2 in the real coordinates of the stored arrays (a spectral entry
+ * is two reals; gradients of spectra are returned as d/dRe + i d/dIm).  The
+ * control is the activity: the solver's activity field (qs_solver_set_activity;
+ * the uniform alpha when none is set) or, with alpha_t, a uniform activity per
+ * step.  All pointers are device memory; spectra in the reference layout. */
+/* nsteps fused steps from the current state, keeping the state every
+ * ckpt_every steps (<= 0: only the initial state) and the final state.
+ * alpha_t: [nsteps] uniform activity of each step, or NULL. */
+int qs_adj_forward(qs_solver* s, int nsteps, int ckpt_every, const Qreal* alpha_t);
+/* Reverse sweep of the last qs_adj_forward.  gW, gR1, gR2: dL/d(final state).
+ * Outputs (each may be NULL): abar = dL/dRa [Ny*Nx]; abar_t = dL/dalpha_t
+ * [nsteps]; uW, uR1, uR2 = dL/d(initial state).  Every segment is recomputed
+ * from its checkpoint, recording each stage's inverse set (7 spectra per
+ * stage), then reversed.  Leaves the solver at the final state. */
+int qs_adj_backward(qs_solver* s, const Qcomp* gW, const Qcomp* gR1, const Qcomp* gR2,
+                    Qreal* abar, Qreal* abar_t, Qcomp* uW, Qcomp* uR1, Qcomp* uR2);
+/* Loss of a vorticity spectrum W and dL/dW: kind 0 = 1/2 mean(omega^2) with
+ * omega = BwdTrans(W); kind 1 = 1/2 mean(u^2 + v^2) with (u, v) from vel_func
+ * (QFldfuncs.cuh:204-224).  *loss is host memory; synchronises the stream. */
+int qs_adj_loss(const Mesh* mesh, int kind, const Qcomp* W, Qreal* loss, Qcomp* Wbar);
+/* Device bytes held by the adjoint of s. */
+int qs_adj_memory(qs_solver* s, size_t* bytes);
+/* Frees the adjoint's device memory (qs_solver_destroy also does). */
+int qs_adj_release(qs_solver* s);
+
 /* ---- host helpers (no GPU) ---------------------------------------------- */
 /* splitmix64 uniform [0,1) stream: u_k = (x_k >> 11) * 2^-53. */
 int qs_synth_uniform(uint64_t seed, Qreal lo, Qreal hi, size_t n, Qreal* host_out);

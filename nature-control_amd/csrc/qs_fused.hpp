@@ -54,6 +54,12 @@ int launch_adj_row(int nfix, const RunDesc* runs, const int2* work, int nwork, i
 int launch_adj_col(int nfix, int amode, const RunDesc* runs, const int2* work, int nwork, int grid,
                    const double2* tw1024, hipStream_t st);
 
+// out[n] = sum_y arow[n*Ny + y] (fixed-order reduction), n < nsteps.
+int row_sums_launch(const double* arow, int nsteps, int Ny, double* out, hipStream_t st);
+// Loss of a vorticity spectrum W (reference layout) and its gradient; kind 0:
+// 1/2 mean(omega^2), kind 1: 1/2 mean(u^2 + v^2).  Synchronises the mesh stream.
+int loss_launch(const Mesh* m, int kind, const double2* W, double* loss, double2* Wbar);
+
 int wave_fft_test(int n, int sign, const double2* in, double2* out, const double2* tw1024, hipStream_t st);
 
 inline int pitch_of(int Nxh) { return (Nxh + 7) & ~7; }

@@ -12,39 +12,10 @@
 #include <utility>
 #include <vector>
 
-#include "qs_fused.hpp"
+#include "qs_solver.hpp"
 
-struct qs_solver {
-  Mesh* mesh = nullptr;
-  qs_params p{};
-  qs::Coef coef;
-  int pitch = 0;
-  // state [Ny][pitch]
-  double2* old_[3] = {nullptr, nullptr, nullptr};   // w, r1, r2
-  double2* new_[3] = {nullptr, nullptr, nullptr};   // stage-1 partial sums
-  double2* mix[7] = {};                              // mixed arrays [Ny][pitch]
-  double* Ra = nullptr;                              // activity field or null
-  double* w00 = nullptr;                             // W(0,0) of the stage
-  int* flag = nullptr;                               // non-finite flag
-  qs::RunDesc* d_desc = nullptr;
-  int2* d_work = nullptr;
-  int nrow = 0, ncol = 0;
-  bool mixed_valid = false;
-  // unfused path (reference call granularity, reference layout)
-  Field* F[21] = {};
-  double* IFt[4] = {};   // IFw, IFwh, IFq, IFqh
-  Qcomp* uold[3] = {};
-  Qcomp* unew[3] = {};
-  bool unfused_ready = false;
-  // per-kernel HIP-event timing (qs_solver_profile)
-  bool prof = false;
-  std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
-};
 
 namespace qs {
-namespace {
-
 // Descriptor of this solver's run (device copy in S->d_desc) and the work
 // lists of its row / column launches.
 RunDesc make_desc(qs_solver* S) {
@@ -66,6 +37,11 @@ RunDesc make_desc(qs_solver* S) {
   d.advect_w = S->p.advect_w;
   return d;
 }
+
+}  // namespace qs
+
+namespace qs {
+namespace {
 
 int upload_desc(qs_solver* S) {
   const RunDesc d = make_desc(S);
@@ -224,6 +200,7 @@ int qs_solver_create(Mesh* mesh, const qs_params* params, qs_solver** out) {
 int qs_solver_destroy(qs_solver* S) {
   if (!S) return QS_OK;
   (void)hipDeviceSynchronize();   // the Mesh may already be gone: do not touch it
+  qs_adj_release(S);
   for (int k = 0; k < 3; ++k) {
     if (S->old_[k]) (void)hipFree(S->old_[k]);
     if (S->new_[k]) (void)hipFree(S->new_[k]);

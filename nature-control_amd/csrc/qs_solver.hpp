@@ -1,0 +1,71 @@
+// qs_solver.hpp -- the solver object shared by the forward (qs_solver.hip) and
+// the discrete adjoint (qs_adjoint_host.hip).
+#pragma once
+
+#include <utility>
+#include <vector>
+
+#include "qs_fused.hpp"
+
+namespace qs {
+
+// Device memory of the checkpointed discrete adjoint of one solver.
+struct AdjState {
+  int nsteps = 0;        // horizon of the last qs_adj_forward
+  int k = 0;             // checkpoint interval (steps per segment)
+  int kcap = 0;          // segment length the stage slots are sized for
+  int ncap = 0;          // horizon the per-step arrays are sized for
+  bool sched = false;    // per-step uniform activity (alpha_t) in use
+  double* alpha_t = nullptr;                 // [nsteps]
+  std::vector<double2*> ckpt;                // 3 per checkpoint, [Ny][pitch]
+  std::vector<double2*> slots;               // 7 per recorded stage, (2 kcap + 1) stages
+  double* w00s = nullptr;                    // W(0,0) per recorded stage
+  double2* lam_[3] = {};                     // adjoint state
+  double2* acc_[3] = {};
+  double* abar = nullptr;                    // [Ny][Nx]
+  double* arow = nullptr;                    // [nsteps][Ny]
+  double* w00bar = nullptr;                  // [Ny]
+  double2* scratch = nullptr;                // [grid][6][1024]
+  int grid = 0;                              // row adjoint blocks
+  RunDesc* d_desc = nullptr;                 // device descriptors of one segment / the forward
+  int ndesc = 0;
+  std::vector<RunDesc> h_desc;
+  size_t bytes = 0;                          // device bytes held
+};
+
+}  // namespace qs
+
+struct qs_solver {
+  Mesh* mesh = nullptr;
+  qs_params p{};
+  qs::Coef coef;
+  int pitch = 0;
+  // state [Ny][pitch]
+  double2* old_[3] = {nullptr, nullptr, nullptr};   // w, r1, r2
+  double2* new_[3] = {nullptr, nullptr, nullptr};   // stage-1 partial sums
+  double2* mix[7] = {};                              // mixed arrays [Ny][pitch]
+  double* Ra = nullptr;                              // activity field or null
+  double* w00 = nullptr;                             // W(0,0) of the stage
+  int* flag = nullptr;                               // non-finite flag
+  qs::RunDesc* d_desc = nullptr;
+  int2* d_work = nullptr;
+  int nrow = 0, ncol = 0;
+  bool mixed_valid = false;
+  // unfused path (reference call granularity, reference layout)
+  Field* F[21] = {};
+  double* IFt[4] = {};   // IFw, IFwh, IFq, IFqh
+  Qcomp* uold[3] = {};
+  Qcomp* unew[3] = {};
+  bool unfused_ready = false;
+  // per-kernel HIP-event timing (qs_solver_profile)
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+  // discrete adjoint (qs_adj_*): checkpoints, recorded stages, adjoint state
+  qs::AdjState* adj = nullptr;
+};
+
+namespace qs {
+// Descriptor of a solver's run with the solver's own arrays (the plain step).
+RunDesc make_desc(qs_solver* S);
+}  // namespace qs

@@ -142,6 +142,11 @@ _SIGS = {
     "qs_debug_wave_fft": [_P, _I, _I, _P, _P],
     "qs_solver_profile_read": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I)],
     "qs_synth_uniform": [ctypes.c_uint64, _D, _D, ctypes.c_size_t, _P],
+    "qs_adj_forward": [_P, _I, _I, _P],
+    "qs_adj_backward": [_P] * 9,
+    "qs_adj_loss": [_P, _I, _P, ctypes.POINTER(_D), _P],
+    "qs_adj_memory": [_P, ctypes.POINTER(ctypes.c_size_t)],
+    "qs_adj_release": [_P],
 }
 _RESTYPE = {"qs_last_error": ctypes.c_char_p, "qs_status_string": ctypes.c_char_p}
 
@@ -371,3 +376,38 @@ class Solver:
         flag = _I(0)
         call("qs_solver_nonfinite", self._s, ctypes.byref(flag))
         return bool(flag.value)
+
+    # ---- discrete adjoint (qs_adj_*) ---------------------------------------
+    def adj_forward(self, nsteps, ckpt_every=0, alpha_t=None):
+        """nsteps steps from the current state with checkpoints every
+        ckpt_every steps; alpha_t: cuda float64 [nsteps] per-step activity."""
+        call("qs_adj_forward", self._s, int(nsteps), int(ckpt_every), ptr(alpha_t))
+
+    def adj_backward(self, gW, gR1, gR2, want_field=True, want_sched=False, want_state=False, nsteps=None):
+        """Reverse sweep: returns (dL/dRa field or None, dL/dalpha_t or None,
+        (dL/dW0, dL/dR1_0, dL/dR2_0) or None)."""
+        m = self.mesh
+        ab = torch.empty(m.phys_shape, dtype=torch.float64, device="cuda") if want_field else None
+        at = torch.empty(nsteps, dtype=torch.float64, device="cuda") if want_sched else None
+        u = [m.spec() for _ in range(3)] if want_state else [None, None, None]
+        call("qs_adj_backward", self._s, ptr(gW), ptr(gR1), ptr(gR2), ptr(ab), ptr(at), *(ptr(t) for t in u))
+        return ab, at, (u if want_state else None)
+
+    def adj_memory(self):
+        b = ctypes.c_size_t(0)
+        call("qs_adj_memory", self._s, ctypes.byref(b))
+        return b.value
+
+    def adj_release(self):
+        call("qs_adj_release", self._s)
+
+
+ENSTROPHY, KINETIC = 0, 1
+
+
+def adj_loss(mesh, W, kind=ENSTROPHY):
+    """(loss, dL/dW) of a vorticity spectrum (qs_adj_loss)."""
+    L = _D(0.0)
+    Wb = mesh.spec()
+    call("qs_adj_loss", mesh.handle, int(kind), ptr(W), ctypes.byref(L), ptr(Wb))
+    return L.value, Wb
