@@ -38,6 +38,10 @@ def parse():
                     help="reload the initial state every R steps (inside the timed region)")
     ap.add_argument("--unfused", action="store_true",
                     help="time the per-op (reference call structure) step instead")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="also time the discrete adjoint workload (256^2, 200 steps, checkpoints every 20)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the discrete adjoint workload and print its line")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
                          "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -77,6 +81,56 @@ def traffic_from_profile(kernel, N):
     return int(float(vals[key]))
 
 
+def adjoint_bench(reps=3):
+    """Gradient of L = 1/2 mean(omega^2) after 200 IFRK2 steps at 256^2 FP64
+    (dt = 1e-3) with respect to the activity field alpha(x) = 1 + 0.1 eta
+    (|k| <= 6, seed 7), checkpoints every 20 steps, through the
+    torch.autograd.Function over the C-ABI.  grad-steps/s = 200 / (time of the
+    forward with checkpoints + loss + backward); cost ratio = that time over
+    the plain fused forward of the same 200 steps."""
+    import numpy as np
+    import torch
+    import qsolver as Q
+    from qsolver import synth, torch_ops as T
+    N, n, dt, k = 256, 200, 1e-3, 20
+    mesh = Q.Mesh(N, N, 2 * np.pi)
+    s = Q.Solver(mesh, Q.params(dt=dt, **CONFIG))
+    q11, q12 = synth.bandlimited_q(N, N, seed=2)
+    W0, R10, R20 = mesh.spec(), mesh.spec(), mesh.spec()
+    mesh.FwdTrans(R10, torch.from_numpy(q11).cuda())
+    mesh.FwdTrans(R20, torch.from_numpy(q12).cuda())
+    alpha0 = torch.from_numpy(synth.activity_field(N, N, seed=7)).cuda()
+
+    def grad_once():
+        alpha = alpha0.clone().requires_grad_(True)
+        W, _, _ = T.rollout(s, alpha, W0, R10, R20, n, k)
+        L = T.spectral_loss(mesh, W, Q.ENSTROPHY)
+        L.backward()
+        return alpha.grad
+
+    def fwd_once():
+        s.set_activity(alpha0)
+        s.set_state(W0, R10, R20)
+        s.step(n)
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+    grad_once(), fwd_once()      # warm-up
+    tg, tf = timed(grad_once), timed(fwd_once)
+    return {"workload": "dL/dalpha(x), L = 1/2 mean(omega^2) after 200 IFRK2 steps, 256x256 FP64, dt=1e-3, "
+                        "alpha = 1 + 0.1 eta (|k|<=6, seed 7), checkpoint every 20 steps",
+            "grad_steps_per_s": round(n / tg, 1), "grad_ms": round(tg * 1e3, 2),
+            "forward_steps_per_s": round(n / tf, 1), "cost_ratio": round(tg / tf, 3),
+            "adjoint_device_bytes": s.adj_memory(), "timing": f"best of {reps}, wall clock between syncs"}
+
+
 def cpu_baseline(N, dt, nsteps, q11, q12):
     """The FP64 CPU oracle on this host's cores, a bounded sample of the same
     workload (nsteps IFRK2 steps from the same initial state)."""
@@ -104,6 +158,9 @@ def main():
     args = parse()
     if args.copy_calibrate:
         copy_calibrate(args.copy_calibrate)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_bench()}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -177,6 +234,7 @@ def main():
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profile(dom, N)}
     step_bytes = 68 * H
+    adj = adjoint_bench() if args.adjoint else None
     cpu = None
     if args.cpu_steps > 0:
         cpu = cpu_baseline(N, dt, args.cpu_steps, q11, q12)
@@ -204,6 +262,7 @@ def main():
                      "frac_of_peak": round(step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
                      "canonical_101H_frac": round(101 * H * steps_per_s / 1e9 / HBM_PEAK_GBS, 4)},
         "finite": finite,
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
 

@@ -255,6 +255,20 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     return exp(Lk * dt);
   };
 
+  // W(0,0) enters every row's dy u + dx v at bin 0: its adjoint is the sum of
+  // the row adjoints' parts (fixed-order block reduction; thread 0 owns the
+  // element i = 0, j = 0 of the first tile)
+  double w00sum = 0.0;
+  if (AMODE != AMODE_START && FLD == 0 && i0 == 0) {   // block-uniform
+    double s = 0.0;
+    for (int y = threadIdx.x; y < NY; y += 256) s += gload(a.w00bar + y);
+    sm[threadIdx.x] = make_double2(s, 0.0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int t = 0; t < 256; ++t) w00sum += sm[t].x;
+    __syncthreads();
+  }
+
   double2 st[16];
   if (AMODE == AMODE_START) {
     T.load(a.lam_[FLD]);
@@ -282,7 +296,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
           const int j = lp + LPC * m;
           const double ky = kyj(j);
           const double k2 = kx * kx + ky * ky;
-          const double ik2 = (i == 0 && j == 0) || !iok ? 0.0 : 1.0 / k2;
+          const double ik2 = (i == 0 && j == 0) || !iok ? 0.0 : recip(k2);
           st[m] = make_double2(st[m].x + ik2 * v[m].x, st[m].y + ik2 * v[m].y);
         }
       }
@@ -295,19 +309,11 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
           const int j = lp + LPC * m;
           const double ky = kyj(j);
           const double k2 = kx * kx + ky * ky;
-          const double t = (i == 0 && j == 0) || !iok ? 0.0 : ky / k2;
+          const double t = (i == 0 && j == 0) || !iok ? 0.0 : ky * recip(k2);
           st[m] = make_double2(st[m].x + t * v[m].y, st[m].y - t * v[m].x);
         }
       }
-      if (i0 == 0 && wave == 0) {   // W(0,0) enters every row's dy u + dx v (row adjoint parts)
-        double s = 0.0;
-        if (T.g == 0) {
-#pragma unroll
-          for (int r = 0; r < 16; ++r) s += gload(a.w00bar + lp + LPC * r);
-        }
-        s = wave_sum(s);
-        if (T.g == 0 && lp == 0) st[0].x += s;
-      }
+      if (i0 == 0 && threadIdx.x == 0) st[0].x += w00sum;   // element (i = 0, j = 0)
     } else {
       const int m0 = FLD == 1 ? 3 : 5;
       T.stage_in(a.mix[m0], st);

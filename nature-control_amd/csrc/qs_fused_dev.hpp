@@ -36,6 +36,14 @@ __device__ __forceinline__ void stage_twiddles(double2* twq, const double2* tw10
   for (int m = tid; m < kTwq; m += nt) twq[m] = tw1024[m];
 }
 
+// 1/x for x > 0 (hardware reciprocal + two Newton steps, within an ulp or
+// two of the quotient): far fewer live registers than an IEEE division.
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * (2.0 - x * r);
+  return r * (2.0 - x * r);
+}
+
 // Row staging: rows row0 .. row0+NV-1 of the 7 arrays arr[0..6] (x-spectral,
 // H = N/2+1 bins each) into sm[t*RH + vec*H + k].  RH = 512 + NV <= 576 < 2*384:
 // two slots per thread and array, all 14 loads in flight before the first LDS
