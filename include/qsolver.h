@@ -241,6 +241,40 @@ int qs_adj_memory(qs_solver* s, size_t* bytes);
 /* Frees the adjoint's device memory (qs_solver_destroy also does). */
 int qs_adj_release(qs_solver* s);
 
+/* ---- ensembles of independent runs ---------------------------------------
+ * No reference counterpart (the reference runs one mesh per process).  n runs
+ * of square meshes N[r] (powers of two, 16..1024) with their own parameters
+ * (activity alpha, friction Rf, ...), advanced together: per mesh size, every
+ * fused kernel covers all runs of that size; sizes run as parallel branches of
+ * a captured hipGraph.  Each run's results equal those of its own qs_solver. */
+typedef struct qs_ensemble qs_ensemble;
+typedef struct qs_ensemble_config {
+  int graph_steps;     /* steps per captured graph launch (0: direct launches) */
+  int concurrent;      /* 1: size classes as parallel branches */
+  int order;           /* work lists: 0 run-major, 1 job-major (interleaved runs) */
+  int row_blocks[8];   /* persistent blocks of the row kernel per size class, 0 = one per job */
+  int col_blocks[8];   /* same for the column kernels (classes in ascending size) */
+} qs_ensemble_config;
+int qs_ensemble_create(int n, const int* N, const qs_params* params, qs_ensemble** out);
+int qs_ensemble_destroy(qs_ensemble* e);
+int qs_ensemble_set_stream(qs_ensemble* e, void* hip_stream);
+int qs_ensemble_size(const qs_ensemble* e, int* n);
+/* The run's solver (for qs_adj_* or per-run stepping); owned by the ensemble. */
+int qs_ensemble_solver(qs_ensemble* e, int run, qs_solver** out);
+int qs_ensemble_set_state(qs_ensemble* e, int run, const Qcomp* w, const Qcomp* r1, const Qcomp* r2);
+int qs_ensemble_get_state(qs_ensemble* e, int run, Qcomp* w, Qcomp* r1, Qcomp* r2);
+int qs_ensemble_set_activity(qs_ensemble* e, int run, const Qreal* Ra);
+/* nsteps IFRK2 steps of every run. */
+int qs_ensemble_step(qs_ensemble* e, int nsteps);
+int qs_ensemble_get_config(const qs_ensemble* e, qs_ensemble_config* cfg);
+int qs_ensemble_set_config(qs_ensemble* e, const qs_ensemble_config* cfg);
+/* Autotuner: coordinate descent over branch concurrency, work-list order and
+ * per-class block counts, each candidate timed over trial_steps directly
+ * launched steps (the graph depth is kept; no graph is captured while tuning);
+ * the states are saved and restored.  Sets and returns the fastest
+ * configuration and its time per (directly launched) ensemble step in ms. */
+int qs_ensemble_tune(qs_ensemble* e, int trial_steps, qs_ensemble_config* best, double* best_ms);
+
 /* ---- host helpers (no GPU) ---------------------------------------------- */
 /* splitmix64 uniform [0,1) stream: u_k = (x_k >> 11) * 2^-53. */
 int qs_synth_uniform(uint64_t seed, Qreal lo, Qreal hi, size_t n, Qreal* host_out);

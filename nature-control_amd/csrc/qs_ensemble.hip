@@ -1,0 +1,424 @@
+// qs_ensemble.hip -- many independent IFRK2 runs of uneven mesh sizes and
+// parameters advanced together; a captured hipGraph per block of steps and a
+// project-written autotuner for the launch shapes.
+//
+// No reference counterpart: the reference advances one run per process
+// (InertialFriction/friction/friction.cu).  Each run is a qs_solver (its own
+// state, mixed arrays, parameters); runs of one mesh size form a size class.
+// One ensemble step launches, per class, the fused kernels of qs_fused.hip
+// over a work list of (run, job) pairs -- row pass, column stage 1, row pass,
+// column stage 2 -- so every kernel of a class covers all its runs.  The
+// classes are independent: with `concurrent` they are parallel branches of the
+// captured graph (fork / join through events), so small classes fill the CUs
+// that large ones leave idle.  The kernels are persistent: a launch of B blocks
+// loops over the work list (B = one per job, or fewer).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "qs_solver.hpp"
+
+namespace qs {
+namespace {
+
+struct EnsClass {
+  int N = 0;
+  Mesh* mesh = nullptr;
+  std::vector<int> members;
+  int2* d_row = nullptr;
+  int2* d_col = nullptr;
+  int nrow = 0, ncol = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t done = nullptr;       // join event of direct launches
+  hipEvent_t cap_done = nullptr;   // join event used only inside graph captures
+  bool valid = false;
+};
+
+}  // namespace
+}  // namespace qs
+
+struct qs_ensemble {
+  std::vector<qs_solver*> runs;
+  std::vector<qs::EnsClass> cls;
+  std::vector<int> cls_of;                 // run -> class index
+  qs::RunDesc* d_desc = nullptr;           // [n] run descriptors
+  hipStream_t stream = nullptr;            // the ensemble's stream (null = default)
+  hipEvent_t fork = nullptr;               // fork event of direct launches
+  hipEvent_t cap_fork = nullptr;           // fork event used only inside graph captures
+  hipStream_t cap = nullptr;               // graph capture stream
+  qs_ensemble_config cfg{};
+  hipGraphExec_t exec = nullptr;
+  int exec_steps = 0;
+  int order_built = -1;
+};
+
+namespace qs {
+namespace {
+
+int upload_descs(qs_ensemble* E) {
+  std::vector<RunDesc> h(E->runs.size());
+  for (size_t r = 0; r < E->runs.size(); ++r) h[r] = make_desc(E->runs[r]);
+  QS_HIP(hipMemcpyAsync(E->d_desc, h.data(), sizeof(RunDesc) * h.size(), hipMemcpyHostToDevice, E->stream));
+  QS_HIP(hipStreamSynchronize(E->stream));   // h is a local
+  return QS_OK;
+}
+
+// Work lists of every class in the configured order.
+int build_work(qs_ensemble* E) {
+  if (E->order_built == E->cfg.order) return QS_OK;
+  for (auto& c : E->cls) {
+    std::vector<int2> row, col;
+    const RunDesc d0 = make_desc(E->runs[c.members[0]]);
+    const int nr = row_jobs(d0), nc = col_jobs(d0);
+    if (E->cfg.order == 0) {
+      for (int r : c.members) {
+        for (int j = 0; j < nr; ++j) row.push_back(make_int2(r, j));
+        for (int j = 0; j < nc; ++j) col.push_back(make_int2(r, j));
+      }
+    } else {
+      for (int j = 0; j < nr; ++j)
+        for (int r : c.members) row.push_back(make_int2(r, j));
+      for (int j = 0; j < nc; ++j)
+        for (int r : c.members) col.push_back(make_int2(r, j));
+    }
+    if (!c.d_row) {
+      if (hipMalloc((void**)&c.d_row, sizeof(int2) * row.size()) != hipSuccess ||
+          hipMalloc((void**)&c.d_col, sizeof(int2) * col.size()) != hipSuccess)
+        return fail(QS_ERR_OOM, "ensemble work lists");
+    }
+    QS_HIP(hipMemcpy(c.d_row, row.data(), sizeof(int2) * row.size(), hipMemcpyHostToDevice));
+    QS_HIP(hipMemcpy(c.d_col, col.data(), sizeof(int2) * col.size(), hipMemcpyHostToDevice));
+    c.nrow = (int)row.size();
+    c.ncol = (int)col.size();
+  }
+  E->order_built = E->cfg.order;
+  return QS_OK;
+}
+
+int blocks(int want, int nwork) { return want > 0 ? std::min(want, nwork) : nwork; }
+
+// One IFRK2 step of class c on stream st.
+int class_step(qs_ensemble* E, int ci, hipStream_t st) {
+  EnsClass& c = E->cls[ci];
+  const double2* tw = c.mesh->priv->tw1024;
+  const int rb = blocks(E->cfg.row_blocks[ci], c.nrow), cb = blocks(E->cfg.col_blocks[ci], c.ncol);
+  QS_TRY(launch_fused_row(c.N, E->d_desc, c.d_row, c.nrow, rb, tw, st));
+  QS_TRY(launch_fused_col(c.N, MODE_S1, E->d_desc, c.d_col, c.ncol, cb, tw, st));
+  QS_TRY(launch_fused_row(c.N, E->d_desc, c.d_row, c.nrow, rb, tw, st));
+  QS_TRY(launch_fused_col(c.N, MODE_S2, E->d_desc, c.d_col, c.ncol, cb, tw, st));
+  return QS_OK;
+}
+
+// One step of every class on stream `main`: parallel branches (fork / join
+// through events) or all on `main`.  Captures use their own events, so no
+// event is shared between a graph and directly launched work.
+int ensemble_step_direct(qs_ensemble* E, hipStream_t main, bool capturing) {
+  const int nc = (int)E->cls.size();
+  if (!E->cfg.concurrent || nc == 1) {
+    for (int ci = 0; ci < nc; ++ci) QS_TRY(class_step(E, ci, main));
+    return QS_OK;
+  }
+  hipEvent_t fork = capturing ? E->cap_fork : E->fork;
+  QS_HIP(hipEventRecord(fork, main));
+  for (int ci = 0; ci < nc; ++ci) {
+    EnsClass& c = E->cls[ci];
+    QS_HIP(hipStreamWaitEvent(c.side, fork, 0));
+    QS_TRY(class_step(E, ci, c.side));
+    QS_HIP(hipEventRecord(capturing ? c.cap_done : c.done, c.side));
+  }
+  for (auto& c : E->cls) QS_HIP(hipStreamWaitEvent(main, capturing ? c.cap_done : c.done, 0));
+  return QS_OK;
+}
+
+// Destroys the executable graph once no launch of it can still be running.
+void drop_graph(qs_ensemble* E) {
+  if (E->exec) {
+    (void)hipStreamSynchronize(E->stream);
+    (void)hipGraphExecDestroy(E->exec);
+  }
+  E->exec = nullptr;
+  E->exec_steps = 0;
+}
+
+// Captured on the ensemble's own capture stream (the null stream cannot be
+// captured); the graph is then launched on the ensemble's stream.
+int capture_graph(qs_ensemble* E, int G) {
+  drop_graph(E);
+  hipGraph_t g = nullptr;
+  QS_HIP(hipStreamBeginCapture(E->cap, hipStreamCaptureModeThreadLocal));
+  int st = QS_OK;
+  for (int k = 0; k < G && st == QS_OK; ++k) st = ensemble_step_direct(E, E->cap, true);
+  const hipError_t e = hipStreamEndCapture(E->cap, &g);
+  if (st != QS_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return fail(QS_ERR_HIP, "ensemble graph capture: %s", hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&E->exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(QS_ERR_HIP, "ensemble graph instantiate: %s", hipGetErrorString(ei));
+  E->exec_steps = G;
+  return QS_OK;
+}
+
+int prologues(qs_ensemble* E) {
+  for (size_t ci = 0; ci < E->cls.size(); ++ci) {
+    EnsClass& c = E->cls[ci];
+    if (c.valid) continue;
+    QS_TRY(launch_fused_col(c.N, MODE_PRO, E->d_desc, c.d_col, c.ncol, blocks(E->cfg.col_blocks[ci], c.ncol),
+                            c.mesh->priv->tw1024, E->stream));
+    c.valid = true;
+  }
+  return QS_OK;
+}
+
+int ensemble_steps(qs_ensemble* E, int nsteps) {
+  QS_TRY(build_work(E));
+  QS_TRY(prologues(E));
+  const int G = E->cfg.graph_steps;
+  int done = 0;
+  if (G > 0 && nsteps >= G) {
+    if (E->exec_steps != G) QS_TRY(capture_graph(E, G));
+    for (; done + G <= nsteps; done += G) QS_HIP(hipGraphLaunch(E->exec, E->stream));
+  }
+  for (; done < nsteps; ++done) QS_TRY(ensemble_step_direct(E, E->stream, false));
+  return QS_OK;
+}
+
+void ensemble_free(qs_ensemble* E) {
+  if (!E) return;
+  (void)hipDeviceSynchronize();
+  drop_graph(E);
+  for (auto* s : E->runs) qs_solver_destroy(s);
+  for (auto& c : E->cls) {
+    if (c.d_row) (void)hipFree(c.d_row);
+    if (c.d_col) (void)hipFree(c.d_col);
+    if (c.side) (void)hipStreamDestroy(c.side);
+    if (c.done) (void)hipEventDestroy(c.done);
+    if (c.cap_done) (void)hipEventDestroy(c.cap_done);
+    qs_mesh_destroy(c.mesh);
+  }
+  if (E->fork) (void)hipEventDestroy(E->fork);
+  if (E->cap_fork) (void)hipEventDestroy(E->cap_fork);
+  if (E->cap) (void)hipStreamDestroy(E->cap);
+  if (E->d_desc) (void)hipFree(E->d_desc);
+  delete E;
+}
+
+int check_run(const qs_ensemble* E, int r) {
+  if (!E) return fail(QS_ERR_INVALID_ARG, "null ensemble");
+  if (r < 0 || r >= (int)E->runs.size()) return fail(QS_ERR_INVALID_ARG, "run %d of %zu", r, E->runs.size());
+  return QS_OK;
+}
+
+}  // namespace
+}  // namespace qs
+
+using namespace qs;
+
+extern "C" {
+
+int qs_ensemble_create(int n, const int* N, const qs_params* params, qs_ensemble** out) {
+  if (!out || !N || !params || n <= 0) return fail(QS_ERR_INVALID_ARG, "qs_ensemble_create: bad arguments");
+  *out = nullptr;
+  qs_ensemble* E = new qs_ensemble();
+  auto bail = [&](int st) { ensemble_free(E); return st; };
+  // size classes in ascending mesh size
+  std::vector<int> sizes(N, N + n);
+  std::sort(sizes.begin(), sizes.end());
+  sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+  if (sizes.size() > 8) return bail(fail(QS_ERR_INVALID_ARG, "qs_ensemble_create: more than 8 mesh sizes"));
+  for (int s : sizes) {
+    EnsClass c;
+    c.N = s;
+    const int st = qs_mesh_create(s, s, 2 * M_PI, 2 * M_PI, 16, &c.mesh);
+    E->cls.push_back(c);
+    if (st != QS_OK) return bail(st);
+    if (hipStreamCreateWithFlags(&E->cls.back().side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&E->cls.back().done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&E->cls.back().cap_done, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(QS_ERR_HIP, "qs_ensemble_create: streams"));
+  }
+  if (hipEventCreateWithFlags(&E->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&E->cap_fork, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&E->cap, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(QS_ERR_HIP, "qs_ensemble_create: events"));
+  E->runs.assign(n, nullptr);
+  E->cls_of.assign(n, 0);
+  for (int r = 0; r < n; ++r) {
+    const int ci = (int)(std::lower_bound(sizes.begin(), sizes.end(), N[r]) - sizes.begin());
+    E->cls_of[r] = ci;
+    E->cls[ci].members.push_back(r);
+    const int st = qs_solver_create(E->cls[ci].mesh, &params[r], &E->runs[r]);
+    if (st != QS_OK) return bail(st);
+  }
+  if (hipMalloc((void**)&E->d_desc, sizeof(RunDesc) * n) != hipSuccess)
+    return bail(fail(QS_ERR_OOM, "qs_ensemble_create: descriptors"));
+  E->cfg.graph_steps = 10;
+  E->cfg.concurrent = 1;
+  E->cfg.order = 0;
+  int st = upload_descs(E);
+  if (st == QS_OK) st = build_work(E);
+  if (st != QS_OK) return bail(st);
+  *out = E;
+  return QS_OK;
+}
+
+int qs_ensemble_destroy(qs_ensemble* E) {
+  ensemble_free(E);
+  return QS_OK;
+}
+
+int qs_ensemble_set_stream(qs_ensemble* E, void* stream) {
+  if (!E) return fail(QS_ERR_INVALID_ARG, "null ensemble");
+  E->stream = (hipStream_t)stream;
+  for (auto& c : E->cls) QS_TRY(qs_mesh_set_stream(c.mesh, stream));
+  drop_graph(E);
+  return QS_OK;
+}
+
+int qs_ensemble_size(const qs_ensemble* E, int* n) {
+  if (!E || !n) return fail(QS_ERR_INVALID_ARG, "null argument");
+  *n = (int)E->runs.size();
+  return QS_OK;
+}
+
+int qs_ensemble_solver(qs_ensemble* E, int r, qs_solver** out) {
+  QS_TRY(check_run(E, r));
+  if (!out) return fail(QS_ERR_INVALID_ARG, "null out");
+  *out = E->runs[r];
+  return QS_OK;
+}
+
+int qs_ensemble_set_state(qs_ensemble* E, int r, const Qcomp* w, const Qcomp* r1, const Qcomp* r2) {
+  QS_TRY(check_run(E, r));
+  QS_TRY(qs_solver_set_state(E->runs[r], w, r1, r2));
+  E->cls[E->cls_of[r]].valid = false;
+  return QS_OK;
+}
+
+int qs_ensemble_get_state(qs_ensemble* E, int r, Qcomp* w, Qcomp* r1, Qcomp* r2) {
+  QS_TRY(check_run(E, r));
+  return qs_solver_get_state(E->runs[r], w, r1, r2);
+}
+
+int qs_ensemble_set_activity(qs_ensemble* E, int r, const Qreal* Ra) {
+  QS_TRY(check_run(E, r));
+  QS_TRY(qs_solver_set_activity(E->runs[r], Ra));
+  return upload_descs(E);
+}
+
+int qs_ensemble_step(qs_ensemble* E, int nsteps) {
+  if (!E) return fail(QS_ERR_INVALID_ARG, "null ensemble");
+  if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
+  return ensemble_steps(E, nsteps);
+}
+
+int qs_ensemble_get_config(const qs_ensemble* E, qs_ensemble_config* cfg) {
+  if (!E || !cfg) return fail(QS_ERR_INVALID_ARG, "null argument");
+  *cfg = E->cfg;
+  return QS_OK;
+}
+
+int qs_ensemble_set_config(qs_ensemble* E, const qs_ensemble_config* cfg) {
+  if (!E || !cfg) return fail(QS_ERR_INVALID_ARG, "null argument");
+  if (cfg->graph_steps < 0 || (cfg->order != 0 && cfg->order != 1))
+    return fail(QS_ERR_INVALID_ARG, "qs_ensemble_set_config: graph_steps %d order %d", cfg->graph_steps, cfg->order);
+  E->cfg = *cfg;
+  drop_graph(E);
+  return build_work(E);
+}
+
+// Coordinate-descent autotuner over the launch configuration: work-list
+// order, branch concurrency and per-class block counts.  Each candidate is
+// timed (HIP events on the ensemble's stream) over `trial_steps` directly
+// launched steps after one warm-up step; no graph is captured while tuning
+// (the configured graph depth is kept, and the graph is captured at the next
+// qs_ensemble_step).  The states are saved first and restored afterwards.
+int qs_ensemble_tune(qs_ensemble* E, int trial_steps, qs_ensemble_config* best_out, double* best_ms) {
+  if (!E) return fail(QS_ERR_INVALID_ARG, "null ensemble");
+  if (trial_steps <= 0) return fail(QS_ERR_INVALID_ARG, "trial_steps must be > 0");
+  const int n = (int)E->runs.size();
+  // save states
+  std::vector<double2*> saved(3 * (size_t)n, nullptr);
+  int st = QS_OK;
+  for (int r = 0; r < n && st == QS_OK; ++r) {
+    qs_solver* S = E->runs[r];
+    const size_t hs = sizeof(double2) * (size_t)S->pitch * S->mesh->Ny;
+    for (int c = 0; c < 3 && st == QS_OK; ++c) {
+      if (hipMalloc((void**)&saved[3 * r + c], hs) != hipSuccess) st = fail(QS_ERR_OOM, "tune: state copy");
+      else if (hipMemcpyAsync(saved[3 * r + c], S->old_[c], hs, hipMemcpyDeviceToDevice, E->stream) != hipSuccess)
+        st = fail(QS_ERR_HIP, "tune: state copy");
+    }
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == QS_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = fail(QS_ERR_HIP, "tune: events");
+  const int graph_steps = E->cfg.graph_steps;
+  auto measure = [&](qs_ensemble_config c, double* ms) -> int {
+    c.graph_steps = 0;                                        // direct launches while tuning
+    QS_TRY(qs_ensemble_set_config(E, &c));
+    QS_TRY(ensemble_steps(E, 1));                             // warm-up
+    QS_HIP(hipEventRecord(e0, E->stream));
+    QS_TRY(ensemble_steps(E, trial_steps));
+    QS_HIP(hipEventRecord(e1, E->stream));
+    QS_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    QS_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / trial_steps;
+    return QS_OK;
+  };
+  qs_ensemble_config best = E->cfg;
+  double bms = 0.0;
+  if (st == QS_OK) st = measure(best, &bms);
+  const int nc = (int)E->cls.size();
+  auto try_cfg = [&](const qs_ensemble_config& c) -> int {
+    double ms = 0.0;
+    const int s = measure(c, &ms);
+    if (s != QS_OK) return s;
+    if (ms < bms) { bms = ms; best = c; }
+    return QS_OK;
+  };
+  for (int sweep = 0; sweep < 2 && st == QS_OK; ++sweep) {
+    for (int v : {0, 1}) {
+      qs_ensemble_config c = best; c.concurrent = v;
+      if (st == QS_OK) st = try_cfg(c);
+    }
+    for (int v : {0, 1}) {
+      qs_ensemble_config c = best; c.order = v;
+      if (st == QS_OK) st = try_cfg(c);
+    }
+    for (int ci = 0; ci < nc && st == QS_OK; ++ci)
+      for (int b : {0, 128, 256, 512, 1024, 2048}) {
+        qs_ensemble_config c = best; c.row_blocks[ci] = b;
+        if (st == QS_OK) st = try_cfg(c);
+        c = best; c.col_blocks[ci] = b;
+        if (st == QS_OK) st = try_cfg(c);
+      }
+  }
+  best.graph_steps = graph_steps;
+  if (st == QS_OK) st = qs_ensemble_set_config(E, &best);
+  // restore states
+  for (int r = 0; r < n; ++r) {
+    qs_solver* S = E->runs[r];
+    const size_t hs = sizeof(double2) * (size_t)S->pitch * S->mesh->Ny;
+    for (int c = 0; c < 3; ++c)
+      if (saved[3 * r + c]) {
+        if (st == QS_OK && hipMemcpyAsync(S->old_[c], saved[3 * r + c], hs, hipMemcpyDeviceToDevice, E->stream) != hipSuccess)
+          st = fail(QS_ERR_HIP, "tune: state restore");
+      }
+  }
+  (void)hipStreamSynchronize(E->stream);
+  for (auto p : saved)
+    if (p) (void)hipFree(p);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (auto& c : E->cls) c.valid = false;
+  if (st == QS_OK) {
+    if (best_out) *best_out = best;
+    if (best_ms) *best_ms = bms;
+  }
+  return st;
+}
+
+}  // extern "C"

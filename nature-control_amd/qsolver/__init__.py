@@ -147,6 +147,18 @@ _SIGS = {
     "qs_adj_loss": [_P, _I, _P, ctypes.POINTER(_D), _P],
     "qs_adj_memory": [_P, ctypes.POINTER(ctypes.c_size_t)],
     "qs_adj_release": [_P],
+    "qs_ensemble_create": [_I, _P, _P, _P],
+    "qs_ensemble_destroy": [_P],
+    "qs_ensemble_set_stream": [_P, _P],
+    "qs_ensemble_size": [_P, ctypes.POINTER(_I)],
+    "qs_ensemble_solver": [_P, _I, _P],
+    "qs_ensemble_set_state": [_P, _I, _P, _P, _P],
+    "qs_ensemble_get_state": [_P, _I, _P, _P, _P],
+    "qs_ensemble_set_activity": [_P, _I, _P],
+    "qs_ensemble_step": [_P, _I],
+    "qs_ensemble_get_config": [_P, _P],
+    "qs_ensemble_set_config": [_P, _P],
+    "qs_ensemble_tune": [_P, _I, _P, ctypes.POINTER(_D)],
 }
 _RESTYPE = {"qs_last_error": ctypes.c_char_p, "qs_status_string": ctypes.c_char_p}
 
@@ -411,3 +423,72 @@ def adj_loss(mesh, W, kind=ENSTROPHY):
     Wb = mesh.spec()
     call("qs_adj_loss", mesh.handle, int(kind), ptr(W), ctypes.byref(L), ptr(Wb))
     return L.value, Wb
+
+
+class EnsembleConfig(ctypes.Structure):
+    """qs_ensemble_config."""
+    _fields_ = [("graph_steps", _I), ("concurrent", _I), ("order", _I),
+                ("row_blocks", _I * 8), ("col_blocks", _I * 8)]
+
+    def as_dict(self):
+        return {"graph_steps": self.graph_steps, "concurrent": self.concurrent, "order": self.order,
+                "row_blocks": list(self.row_blocks), "col_blocks": list(self.col_blocks)}
+
+
+class Ensemble:
+    """Independent IFRK2 runs of uneven square meshes advanced together
+    (qs_ensemble_*).  sizes: mesh size per run; params: qs_params per run."""
+
+    def __init__(self, sizes, params_list):
+        n = len(sizes)
+        self.sizes = list(sizes)
+        Ns = (_I * n)(*sizes)
+        ps = (QsParams * n)(*params_list)
+        self._e = ctypes.c_void_p()
+        call("qs_ensemble_create", n, Ns, ps, ctypes.byref(self._e))
+        self._meshes = {}
+        _live["solver"].add(self)
+
+    def close(self):
+        if getattr(self, "_e", None) and self._e.value:
+            if _lib is not None:
+                _lib.qs_ensemble_destroy(self._e)
+            self._e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def spec(self, r):
+        N = self.sizes[r]
+        return torch.zeros((N, N // 2 + 1), dtype=torch.complex128, device="cuda")
+
+    def set_state(self, r, w, r1, r2):
+        call("qs_ensemble_set_state", self._e, int(r), ptr(w), ptr(r1), ptr(r2))
+
+    def get_state(self, r):
+        out = [self.spec(r) for _ in range(3)]
+        call("qs_ensemble_get_state", self._e, int(r), *(ptr(t) for t in out))
+        return out
+
+    def set_activity(self, r, Ra):
+        call("qs_ensemble_set_activity", self._e, int(r), ptr(Ra))
+
+    def step(self, nsteps=1):
+        call("qs_ensemble_step", self._e, int(nsteps))
+
+    def config(self):
+        c = EnsembleConfig()
+        call("qs_ensemble_get_config", self._e, ctypes.byref(c))
+        return c
+
+    def set_config(self, c):
+        call("qs_ensemble_set_config", self._e, ctypes.byref(c))
+
+    def tune(self, trial_steps=5):
+        c = EnsembleConfig()
+        ms = _D(0.0)
+        call("qs_ensemble_tune", self._e, int(trial_steps), ctypes.byref(c), ctypes.byref(ms))
+        return c, ms.value

@@ -1,0 +1,112 @@
+"""Ensembles of uneven runs (qs_ensemble_*): every run equals its own single
+run bit for bit, under every launch configuration the autotuner can pick."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import qsolver as Q  # noqa: E402
+from qsolver import synth  # noqa: E402
+
+BASE = dict(lam=1.0, Re=1.0, Er=1.0, D_Q=0.05, a_Q=1.0, c_Q=1.0, dt=1e-3)
+SIZES = [32, 64, 32, 16, 64, 128]
+
+
+def run_params(n, seed=3):
+    rng = np.random.default_rng(seed)
+    return [Q.params(alpha=float(rng.uniform(0.5, 2.0)), Rf=float(rng.uniform(0.01, 0.5)), **BASE)
+            for _ in range(n)]
+
+
+def init_state(N, seed):
+    q11, q12 = synth.uniform_q(N, N, seed=seed)
+    mesh = Q.Mesh(N, N, 2 * np.pi)
+    st = [mesh.spec(), mesh.spec(), mesh.spec()]
+    mesh.FwdTrans(st[1], torch.from_numpy(q11).cuda())
+    mesh.FwdTrans(st[2], torch.from_numpy(q12).cuda())
+    return st
+
+
+def single_runs(sizes, ps, inits, nsteps, activity=None):
+    out = []
+    for r, (N, p) in enumerate(zip(sizes, ps)):
+        mesh = Q.Mesh(N, N, 2 * np.pi)
+        s = Q.Solver(mesh, p)
+        s.set_state(*inits[r])
+        if activity is not None and r in activity:
+            s.set_activity(activity[r])
+        s.step(nsteps)
+        out.append([t.clone() for t in s.get_state()])
+        s.close()
+    return out
+
+
+def ensemble_runs(sizes, ps, inits, nsteps, cfg=None, activity=None, tune=False):
+    e = Q.Ensemble(sizes, ps)
+    for r in range(len(sizes)):
+        e.set_state(r, *inits[r])
+    if activity is not None:
+        for r, Ra in activity.items():
+            e.set_activity(r, Ra)
+    if cfg is not None:
+        c = e.config()
+        for k, v in cfg.items():
+            if isinstance(v, list):
+                getattr(c, k)[:] = v
+            else:
+                setattr(c, k, v)
+        e.set_config(c)
+    if tune:
+        best, ms = e.tune(2)
+        assert ms > 0
+    e.step(nsteps)
+    out = [e.get_state(r) for r in range(len(sizes))]
+    e.close()
+    return out
+
+
+@pytest.fixture(scope="module")
+def reference():
+    ps = run_params(len(SIZES))
+    inits = [init_state(N, 1000 + r) for r, N in enumerate(SIZES)]
+    return ps, inits, single_runs(SIZES, ps, inits, 13)
+
+
+CONFIGS = {
+    "default": None,
+    "direct-serial": {"graph_steps": 0, "concurrent": 0},
+    "graph1-jobmajor": {"graph_steps": 1, "order": 1},
+    "few-blocks": {"graph_steps": 4, "row_blocks": [1, 3, 2, 5, 0, 0, 0, 0], "col_blocks": [2, 1, 3, 7, 0, 0, 0, 0]},
+}
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_ensemble_equals_single_runs(reference, name):
+    print(f"[gpu-test] ensemble config {name}", flush=True)
+    ps, inits, want = reference
+    got = ensemble_runs(SIZES, ps, inits, 13, cfg=CONFIGS[name])
+    for r, (g, w) in enumerate(zip(got, want)):
+        for a, b in zip(g, w):
+            assert torch.equal(a, b), (name, r)
+
+
+def test_autotuner_restores_states(reference):
+    print("[gpu-test] ensemble autotune", flush=True)
+    ps, inits, want = reference
+    got = ensemble_runs(SIZES, ps, inits, 13, tune=True)
+    for g, w in zip(got, want):
+        for a, b in zip(g, w):
+            assert torch.equal(a, b)
+
+
+def test_per_run_activity_field(reference):
+    print("[gpu-test] ensemble activity field", flush=True)
+    ps, inits, _ = reference
+    act = {1: torch.from_numpy(synth.activity_field(64, 64, seed=5)).cuda(),
+           3: torch.from_numpy(synth.activity_field(16, 16, seed=6)).cuda()}
+    want = single_runs(SIZES, ps, inits, 7, activity=act)
+    got = ensemble_runs(SIZES, ps, inits, 7, activity=act)
+    for g, w in zip(got, want):
+        for a, b in zip(g, w):
+            assert torch.equal(a, b)
