@@ -40,6 +40,10 @@ def parse():
                     help="time the per-op (reference call structure) step instead")
     ap.add_argument("--adjoint", type=int, default=1,
                     help="also time the discrete adjoint workload (256^2, 200 steps, checkpoints every 20)")
+    ap.add_argument("--ensemble", type=int, default=1,
+                    help="also time the uneven ensemble (64x128^2 + 128x256^2 + 16x512^2, 200 steps)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only run the ensemble workload and print its line")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the discrete adjoint workload and print its line")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -131,6 +135,86 @@ def adjoint_bench(reps=3):
             "adjoint_device_bytes": s.adj_memory(), "timing": f"best of {reps}, wall clock between syncs"}
 
 
+def ensemble_bench(nsteps=200, loop_steps=20, tune_steps=3):
+    """64 runs at 128^2, 128 at 256^2, 16 at 512^2 (FP64 IFRK2, dt = 1e-3),
+    run r: uniform Q in [-0.01, 0.01] (splitmix64 seed 1000 + r), omega = 0,
+    activity ~ U[0.5, 2], friction ~ U[0.01, 0.5] (numpy seed 3).  Aggregate
+    run-steps/s of the ensemble (autotuned launch shapes, graph of 10 steps)
+    against a loop of plain single-run qs_step_ifrk2 calls over the same runs
+    (step-major: every run advances one step before the next step); the loop
+    is timed over fewer steps and reported per run-step."""
+    import numpy as np
+    import torch
+    import qsolver as Q
+    from qsolver import synth
+    sizes = [128] * 64 + [256] * 128 + [512] * 16
+    rng = np.random.default_rng(3)
+    base = {k: v for k, v in CONFIG.items() if k not in ("alpha", "Rf")}
+    ps = [Q.params(alpha=float(rng.uniform(0.5, 2.0)), Rf=float(rng.uniform(0.01, 0.5)), dt=1e-3, **base)
+          for _ in sizes]
+    e = Q.Ensemble(sizes, ps)
+    meshes = {N: Q.Mesh(N, N, 2 * np.pi) for N in set(sizes)}
+    inits = []
+    for r, N in enumerate(sizes):
+        q11, q12 = synth.uniform_q(N, N, seed=1000 + r)
+        m = meshes[N]
+        st = [m.spec(), m.spec(), m.spec()]
+        m.FwdTrans(st[1], torch.from_numpy(q11).cuda())
+        m.FwdTrans(st[2], torch.from_numpy(q12).cuda())
+        inits.append(st)
+        e.set_state(r, *st)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cfg, tuned_ms = e.tune(tune_steps)
+    tune_s = time.perf_counter() - t0
+    for r in range(len(sizes)):
+        e.set_state(r, *inits[r])
+    e.step(10)                                  # warm-up, graph capture
+    for r in range(len(sizes)):
+        e.set_state(r, *inits[r])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.step(nsteps)
+    torch.cuda.synchronize()
+    te = time.perf_counter() - t0
+    # loop of single-run steps over the ensemble's own runs (their solvers)
+    solvers = []
+    for r in range(len(sizes)):
+        sp = ctypes_solver(e, r)
+        solvers.append(sp)
+    for r, S in enumerate(solvers):
+        Q.call("qs_solver_set_state", S, *(Q.ptr(t) for t in inits[r]))
+    for S in solvers:
+        Q.call("qs_step_ifrk2", S, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for S in solvers:
+            Q.call("qs_step_ifrk2", S, 1)
+    torch.cuda.synchronize()
+    tl = time.perf_counter() - t0
+    ens_rate = len(sizes) * nsteps / te
+    loop_rate = len(sizes) * loop_steps / tl
+    H = sum(N * (N // 2 + 1) * 16 for N in sizes)
+    e.close()
+    return {"workload": "64x128^2 + 128x256^2 + 16x512^2 FP64 IFRK2, dt=1e-3, per-run activity U[0.5,2], "
+                        "friction U[0.01,0.5], uniform Q init (seed 1000+r)",
+            "run_steps_per_s": round(ens_rate, 1), "ensemble_steps_per_s": round(nsteps / te, 2),
+            "loop_run_steps_per_s": round(loop_rate, 1), "speedup_vs_loop": round(ens_rate / loop_rate, 2),
+            "alg_GBps_68H": round(68 * H * nsteps / te / 1e9, 1),
+            "canonical_101H_frac": round(101 * H * nsteps / te / 1e9 / HBM_PEAK_GBS, 4),
+            "tuned_config": cfg.as_dict(), "tune_seconds": round(tune_s, 2),
+            "timing": f"{nsteps} ensemble steps vs {loop_steps} loop steps, wall clock between syncs"}
+
+
+def ctypes_solver(e, r):
+    import ctypes
+    import qsolver as Q
+    out = ctypes.c_void_p()
+    Q.call("qs_ensemble_solver", e._e, int(r), ctypes.byref(out))
+    return out
+
+
 def cpu_baseline(N, dt, nsteps, q11, q12):
     """The FP64 CPU oracle on this host's cores, a bounded sample of the same
     workload (nsteps IFRK2 steps from the same initial state)."""
@@ -158,6 +242,9 @@ def main():
     args = parse()
     if args.copy_calibrate:
         copy_calibrate(args.copy_calibrate)
+        return
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench()}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench()}), flush=True)
@@ -235,6 +322,7 @@ def main():
                     "traffic": traffic_from_profile(dom, N)}
     step_bytes = 68 * H
     adj = adjoint_bench() if args.adjoint else None
+    ens = ensemble_bench() if args.ensemble else None
     cpu = None
     if args.cpu_steps > 0:
         cpu = cpu_baseline(N, dt, args.cpu_steps, q11, q12)
@@ -263,6 +351,7 @@ def main():
                      "canonical_101H_frac": round(101 * H * steps_per_s / 1e9 / HBM_PEAK_GBS, 4)},
         "finite": finite,
         "adjoint": adj,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 
