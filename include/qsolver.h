@@ -275,6 +275,19 @@ int qs_ensemble_set_config(qs_ensemble* e, const qs_ensemble_config* cfg);
  * configuration and its time per (directly launched) ensemble step in ms. */
 int qs_ensemble_tune(qs_ensemble* e, int trial_steps, qs_ensemble_config* best, double* best_ms);
 
+/* Batched discrete adjoint of an ensemble whose runs share one mesh size: the
+ * qs_adj_* sequence for every run at once (each launch covers all runs).
+ * alpha_t: device [n][nsteps] uniform activity per run and step, or NULL. */
+int qs_ensemble_adj_forward(qs_ensemble* e, int nsteps, int ckpt_every, const Qreal* alpha_t);
+/* After qs_ensemble_adj_forward: losses[r] (host) of every run's final state
+ * (kind as qs_adj_loss) and their gradients as the seeds of the reverse sweep.
+ * Synchronises. */
+int qs_ensemble_adj_loss(qs_ensemble* e, int kind, Qreal* losses);
+/* Reverse sweep for the sum of the losses.  Outputs (device, may be NULL):
+ * abar_t [n][nsteps] = dL/dalpha_t, abar [n][Ny*Nx] = dL/dRa. */
+int qs_ensemble_adj_backward(qs_ensemble* e, Qreal* abar_t, Qreal* abar);
+int qs_ensemble_adj_memory(qs_ensemble* e, size_t* bytes);
+
 /* ---- host helpers (no GPU) ---------------------------------------------- */
 /* splitmix64 uniform [0,1) stream: u_k = (x_k >> 11) * 2^-53. */
 int qs_synth_uniform(uint64_t seed, Qreal lo, Qreal hi, size_t n, Qreal* host_out);

@@ -523,7 +523,7 @@ __global__ __launch_bounds__(256) void k_row_sums(const double* arow, int Ny, do
 // G = i ky psi and -i kx psi (psi = W/k^2), chained through the conjugate
 // multipliers.  partial[block] = the block's sum of c |P|^2 (fixed order).
 __global__ __launch_bounds__(256) void k_loss(int Nx, int Ny, double dkx, double dky, int kind, const double2* W,
-                                              double2* Wbar, double* partial) {
+                                              int pin, double2* Wbar, int pout, double* partial) {
   __shared__ double red[256];
   const int Nxh = Nx / 2 + 1, t = threadIdx.x;
   const long e = (long)blockIdx.x * 256 + t;
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256) void k_loss(int Nx, int Ny, double dkx, double
     const double kx = dkx * i;
     // G_kind(jj) at column i: W, i ky psi, or -i kx psi
     auto G = [&](int which, int jj) {
-      const double2 w = W[(size_t)jj * Nxh + i];
+      const double2 w = W[(size_t)jj * pin + i];
       if (which == 0) return w;
       const double ky = kyof(jj), k2 = kx * kx + ky * ky;
       const double ip = (i == 0 && jj == 0) ? 0.0 : 1.0 / k2;
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(256) void k_loss(int Nx, int Ny, double dkx, double
       // conj(i ky ip) pu + conj(-i kx ip) pv = -i ky ip pu + i kx ip pv
       wb = make_double2(s * (ky * ip * pu.y - kx * ip * pv.y), s * (-ky * ip * pu.x + kx * ip * pv.x));
     }
-    Wbar[(size_t)j * Nxh + i] = wb;
+    Wbar[(size_t)j * pout + i] = wb;
   }
   red[t] = contrib;
   __syncthreads();
@@ -582,13 +582,13 @@ int row_sums_launch(const double* arow, int nsteps, int Ny, double* out, hipStre
   return QS_OK;
 }
 
-int loss_launch(const Mesh* m, int kind, const double2* W, double* loss, double2* Wbar) {
+int loss_launch(const Mesh* m, int kind, const double2* W, int pin, double* loss, double2* Wbar, int pout) {
   const long n = (long)m->Nxh * m->Ny;
   const int blocks = (int)((n + 255) / 256);
   const hipStream_t st = stream_of(m);
   double* partial = m->phys;   // Nx*Ny >= blocks
   hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, st, m->Nx, m->Ny, 2 * M_PI / m->Lx, 2 * M_PI / m->Ly, kind,
-                     W, Wbar, partial);
+                     W, pin, Wbar, pout, partial);
   QS_CHECK_LAUNCH();
   std::vector<double> h(blocks);
   QS_HIP(hipMemcpyAsync(h.data(), partial, sizeof(double) * blocks, hipMemcpyDeviceToHost, st));

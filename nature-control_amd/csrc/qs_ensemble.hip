@@ -50,6 +50,7 @@ struct qs_ensemble {
   hipGraphExec_t exec = nullptr;
   int exec_steps = 0;
   int order_built = -1;
+  qs::AdjGroup* adj = nullptr;             // batched adjoint (one mesh size)
 };
 
 namespace qs {
@@ -189,6 +190,7 @@ void ensemble_free(qs_ensemble* E) {
   if (!E) return;
   (void)hipDeviceSynchronize();
   drop_graph(E);
+  adj_group_free(E->adj);
   for (auto* s : E->runs) qs_solver_destroy(s);
   for (auto& c : E->cls) {
     if (c.d_row) (void)hipFree(c.d_row);
@@ -419,6 +421,48 @@ int qs_ensemble_tune(qs_ensemble* E, int trial_steps, qs_ensemble_config* best_o
     if (best_ms) *best_ms = bms;
   }
   return st;
+}
+
+// ---- batched discrete adjoint (one mesh size) ------------------------------
+int qs_ensemble_adj_forward(qs_ensemble* E, int nsteps, int ckpt_every, const Qreal* alpha_t) {
+  if (!E) return fail(QS_ERR_INVALID_ARG, "null ensemble");
+  if (E->cls.size() != 1) return fail(QS_ERR_UNSUPPORTED_SIZE, "ensemble adjoint: runs of one mesh size only");
+  if (nsteps <= 0) return fail(QS_ERR_INVALID_ARG, "nsteps must be > 0");
+  if (!E->adj) E->adj = adj_group_new(E->runs);
+  const int k = ckpt_every <= 0 ? nsteps : std::min(ckpt_every, nsteps);
+  QS_TRY(adj_group_forward(E->adj, nsteps, k, alpha_t));
+  E->cls[0].valid = true;   // the mixed arrays hold the final states' inverse sets
+  return QS_OK;
+}
+
+int qs_ensemble_adj_loss(qs_ensemble* E, int kind, Qreal* losses) {
+  if (!E || !losses) return fail(QS_ERR_INVALID_ARG, "null argument");
+  if (kind != 0 && kind != 1) return fail(QS_ERR_INVALID_ARG, "loss kind %d", kind);
+  if (!E->adj || adj_group_steps(E->adj) <= 0) return fail(QS_ERR_STATE, "ensemble adjoint: no forward");
+  for (size_t r = 0; r < E->runs.size(); ++r) QS_TRY(adj_run_seed(E->runs[r], kind, &losses[r]));
+  return QS_OK;
+}
+
+int qs_ensemble_adj_backward(qs_ensemble* E, Qreal* abar_t, Qreal* abar) {
+  if (!E) return fail(QS_ERR_INVALID_ARG, "null ensemble");
+  if (!E->adj || adj_group_steps(E->adj) <= 0) return fail(QS_ERR_STATE, "ensemble adjoint: no forward");
+  QS_TRY(adj_group_backward(E->adj));
+  const int nsteps = adj_group_steps(E->adj);
+  for (size_t r = 0; r < E->runs.size(); ++r) {
+    qs_solver* S = E->runs[r];
+    const Mesh* m = S->mesh;
+    const size_t P = (size_t)m->Nx * m->Ny;
+    if (abar_t) QS_TRY(row_sums_launch(S->adj->arow, nsteps, m->Ny, abar_t + r * (size_t)nsteps, stream_of(m)));
+    if (abar) QS_HIP(hipMemcpyAsync(abar + r * P, S->adj->abar, sizeof(double) * P, hipMemcpyDeviceToDevice, stream_of(m)));
+  }
+  E->cls[0].valid = false;
+  return QS_OK;
+}
+
+int qs_ensemble_adj_memory(qs_ensemble* E, size_t* bytes) {
+  if (!E || !bytes) return fail(QS_ERR_INVALID_ARG, "null argument");
+  *bytes = E->adj ? adj_group_bytes(E->adj) : 0;
+  return QS_OK;
 }
 
 }  // extern "C"

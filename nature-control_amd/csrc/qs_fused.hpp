@@ -56,9 +56,10 @@ int launch_adj_col(int nfix, int amode, const RunDesc* runs, const int2* work, i
 
 // out[n] = sum_y arow[n*Ny + y] (fixed-order reduction), n < nsteps.
 int row_sums_launch(const double* arow, int nsteps, int Ny, double* out, hipStream_t st);
-// Loss of a vorticity spectrum W (reference layout) and its gradient; kind 0:
-// 1/2 mean(omega^2), kind 1: 1/2 mean(u^2 + v^2).  Synchronises the mesh stream.
-int loss_launch(const Mesh* m, int kind, const double2* W, double* loss, double2* Wbar);
+// Loss of a vorticity spectrum W (row pitch pin) and its gradient (row pitch
+// pout); kind 0: 1/2 mean(omega^2), kind 1: 1/2 mean(u^2 + v^2).  Synchronises
+// the mesh stream.
+int loss_launch(const Mesh* m, int kind, const double2* W, int pin, double* loss, double2* Wbar, int pout);
 
 int wave_fft_test(int n, int sign, const double2* in, double2* out, const double2* tw1024, hipStream_t st);
 

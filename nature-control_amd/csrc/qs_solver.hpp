@@ -9,28 +9,23 @@
 
 namespace qs {
 
-// Device memory of the checkpointed discrete adjoint of one solver.
+struct AdjGroup;
+
+// Device memory of the checkpointed discrete adjoint of one run.
 struct AdjState {
-  int nsteps = 0;        // horizon of the last qs_adj_forward
-  int k = 0;             // checkpoint interval (steps per segment)
   int kcap = 0;          // segment length the stage slots are sized for
   int ncap = 0;          // horizon the per-step arrays are sized for
-  bool sched = false;    // per-step uniform activity (alpha_t) in use
-  double* alpha_t = nullptr;                 // [nsteps]
+  int nckpt = 0;         // checkpoints held
   std::vector<double2*> ckpt;                // 3 per checkpoint, [Ny][pitch]
   std::vector<double2*> slots;               // 7 per recorded stage, (2 kcap + 1) stages
   double* w00s = nullptr;                    // W(0,0) per recorded stage
   double2* lam_[3] = {};                     // adjoint state
   double2* acc_[3] = {};
   double* abar = nullptr;                    // [Ny][Nx]
-  double* arow = nullptr;                    // [nsteps][Ny]
+  double* arow = nullptr;                    // [ncap][Ny]
   double* w00bar = nullptr;                  // [Ny]
-  double2* scratch = nullptr;                // [grid][6][1024]
-  int grid = 0;                              // row adjoint blocks
-  RunDesc* d_desc = nullptr;                 // device descriptors of one segment / the forward
-  int ndesc = 0;
-  std::vector<RunDesc> h_desc;
   size_t bytes = 0;                          // device bytes held
+  AdjGroup* solo = nullptr;                  // the group of this run alone (qs_adj_*)
 };
 
 }  // namespace qs
@@ -68,4 +63,15 @@ struct qs_solver {
 namespace qs {
 // Descriptor of a solver's run with the solver's own arrays (the plain step).
 RunDesc make_desc(qs_solver* S);
+}  // namespace qs
+
+namespace qs {
+// Adjoint of a group of runs of one mesh size (qs_adjoint_host.hip).
+AdjGroup* adj_group_new(const std::vector<qs_solver*>& runs);
+void adj_group_free(AdjGroup* G);
+int adj_group_forward(AdjGroup* G, int nsteps, int k, const double* alpha_t);
+int adj_group_backward(AdjGroup* G);
+int adj_group_steps(const AdjGroup* G);
+size_t adj_group_bytes(const AdjGroup* G);
+int adj_run_seed(qs_solver* S, int kind, double* loss);
 }  // namespace qs

@@ -159,6 +159,10 @@ _SIGS = {
     "qs_ensemble_get_config": [_P, _P],
     "qs_ensemble_set_config": [_P, _P],
     "qs_ensemble_tune": [_P, _I, _P, ctypes.POINTER(_D)],
+    "qs_ensemble_adj_forward": [_P, _I, _I, _P],
+    "qs_ensemble_adj_loss": [_P, _I, _P],
+    "qs_ensemble_adj_backward": [_P, _P, _P],
+    "qs_ensemble_adj_memory": [_P, ctypes.POINTER(ctypes.c_size_t)],
 }
 _RESTYPE = {"qs_last_error": ctypes.c_char_p, "qs_status_string": ctypes.c_char_p}
 
@@ -486,6 +490,31 @@ class Ensemble:
 
     def set_config(self, c):
         call("qs_ensemble_set_config", self._e, ctypes.byref(c))
+
+    # ---- batched discrete adjoint (runs of one mesh size) -------------------
+    def adj_forward(self, nsteps, ckpt_every=0, alpha_t=None):
+        """alpha_t: cuda float64 [n, nsteps] per-run per-step activity, or None."""
+        call("qs_ensemble_adj_forward", self._e, int(nsteps), int(ckpt_every), ptr(alpha_t))
+
+    def adj_loss(self, kind=0):
+        """Per-run losses (numpy [n]) of the final states; sets the reverse seeds."""
+        import numpy as np
+        out = np.zeros(len(self.sizes))
+        call("qs_ensemble_adj_loss", self._e, int(kind), out.ctypes.data_as(_P))
+        return out
+
+    def adj_backward(self, nsteps, want_sched=True, want_field=False):
+        n = len(self.sizes)
+        N = self.sizes[0]
+        at = torch.empty((n, nsteps), dtype=torch.float64, device="cuda") if want_sched else None
+        ab = torch.empty((n, N, N), dtype=torch.float64, device="cuda") if want_field else None
+        call("qs_ensemble_adj_backward", self._e, ptr(at), ptr(ab))
+        return at, ab
+
+    def adj_memory(self):
+        b = ctypes.c_size_t(0)
+        call("qs_ensemble_adj_memory", self._e, ctypes.byref(b))
+        return b.value
 
     def tune(self, trial_steps=5):
         c = EnsembleConfig()
