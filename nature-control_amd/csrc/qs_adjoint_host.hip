@@ -60,14 +60,7 @@ void group_free(AdjGroup* G) {
 
 void run_free(AdjState* A) {
   if (!A) return;
-  for (auto p : A->ckpt) (void)hipFree(p);
-  for (auto p : A->slots) (void)hipFree(p);
-  for (void* p : {(void*)A->w00s, (void*)A->abar, (void*)A->arow, (void*)A->w00bar})
-    if (p) (void)hipFree(p);
-  for (int k = 0; k < 3; ++k) {
-    if (A->lam_[k]) (void)hipFree(A->lam_[k]);
-    if (A->acc_[k]) (void)hipFree(A->acc_[k]);
-  }
+  if (A->arena) (void)hipFree(A->arena);
   group_free(A->solo);
   delete A;
 }
@@ -76,7 +69,9 @@ size_t hbytes(const qs_solver* S) { return sizeof(double2) * (size_t)S->pitch * 
 const Mesh* mesh_of(const AdjGroup* G) { return G->runs[0]->mesh; }
 hipStream_t stream_g(const AdjGroup* G) { return stream_of(mesh_of(G)); }
 
-// Per-run memory for a horizon of nsteps with checkpoint interval k.
+// Per-run memory for a horizon of nsteps with checkpoint interval k: one
+// allocation carved into 256-byte aligned arrays (many separate allocations of
+// a few MB each round up to the allocator's granularity).
 int run_prepare(qs_solver* S, int nsteps, int k) {
   const Mesh* m = S->mesh;
   AdjState* A = S->adj;
@@ -90,19 +85,27 @@ int run_prepare(qs_solver* S, int nsteps, int k) {
   A->kcap = k;
   A->ncap = nsteps;
   A->nckpt = nseg + 1;
-  const size_t hs = hbytes(S);
+  const size_t hs = (hbytes(S) + 255) & ~(size_t)255;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t P = (size_t)m->Nx * m->Ny;
+  const size_t total = hs * (3 * (size_t)(nseg + 1) + 7 * (size_t)(2 * k + 1) + 6) +
+                       al(sizeof(double) * (2 * k + 1)) + al(sizeof(double) * P) +
+                       al(sizeof(double) * (size_t)nsteps * m->Ny) + al(sizeof(double) * m->Ny);
+  QS_TRY(dalloc(&A->bytes, &A->arena, total));
+  char* p = (char*)A->arena;
+  auto take = [&](size_t b) { char* q = p; p += b; return q; };
   A->ckpt.assign(3 * (size_t)(nseg + 1), nullptr);
-  for (auto& p : A->ckpt) QS_TRY(dalloc(&A->bytes, &p, hs));
+  for (auto& q : A->ckpt) q = (double2*)take(hs);
   A->slots.assign(7 * (size_t)(2 * k + 1), nullptr);
-  for (auto& p : A->slots) QS_TRY(dalloc(&A->bytes, &p, hs));
-  QS_TRY(dalloc(&A->bytes, &A->w00s, sizeof(double) * (2 * k + 1)));
+  for (auto& q : A->slots) q = (double2*)take(hs);
   for (int c = 0; c < 3; ++c) {
-    QS_TRY(dalloc(&A->bytes, &A->lam_[c], hs));
-    QS_TRY(dalloc(&A->bytes, &A->acc_[c], hs));
+    A->lam_[c] = (double2*)take(hs);
+    A->acc_[c] = (double2*)take(hs);
   }
-  QS_TRY(dalloc(&A->bytes, &A->abar, sizeof(double) * (size_t)m->Nx * m->Ny));
-  QS_TRY(dalloc(&A->bytes, &A->arow, sizeof(double) * (size_t)nsteps * m->Ny));
-  QS_TRY(dalloc(&A->bytes, &A->w00bar, sizeof(double) * m->Ny));
+  A->w00s = (double*)take(al(sizeof(double) * (2 * k + 1)));
+  A->abar = (double*)take(al(sizeof(double) * P));
+  A->arow = (double*)take(al(sizeof(double) * (size_t)nsteps * m->Ny));
+  A->w00bar = (double*)take(al(sizeof(double) * m->Ny));
   return QS_OK;
 }
 

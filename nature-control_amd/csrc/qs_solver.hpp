@@ -24,6 +24,7 @@ struct AdjState {
   double* abar = nullptr;                    // [Ny][Nx]
   double* arow = nullptr;                    // [ncap][Ny]
   double* w00bar = nullptr;                  // [Ny]
+  void* arena = nullptr;                     // the one allocation holding all of the above
   size_t bytes = 0;                          // device bytes held
   AdjGroup* solo = nullptr;                  // the group of this run alone (qs_adj_*)
 };
