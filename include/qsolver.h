@@ -191,6 +191,13 @@ int qs_solver_get_state(qs_solver* s, Qcomp* w, Qcomp* r1, Qcomp* r2);
  * IFh := exp(L dt)), fused into four kernel launches per step (row pass and
  * column stage, twice) plus one column prologue after set_state. */
 int qs_step_ifrk2(qs_solver* s, int nsteps);
+/* nsteps IF-RK4 steps: friction.cu:212-260 with TimeIntegration/RK4.cuh:12-74
+ * (IF = exp(L dt), IFh = exp(L dt/2), as the reference's driver), fused into
+ * four row passes and four column stages per step. */
+int qs_step_rk4(qs_solver* s, int nsteps);
+/* nsteps IF-Euler steps (TimeIntegration/IFE.cuh:15-25, then dealias), fused
+ * into one row pass and one column stage per step. */
+int qs_step_ife(qs_solver* s, int nsteps);
 /* Same step built from the per-op entry points above (reference call
  * granularity); for parity and for timing the unfused structure. */
 int qs_step_ifrk2_unfused(qs_solver* s, int nsteps);
@@ -200,7 +207,9 @@ int qs_solver_nonfinite(qs_solver* s, int* flag);
  * every kernel of qs_step_ifrk2 is bracketed by HIP events on the Mesh's
  * stream; qs_solver_profile_read synchronises and returns the average
  * duration in ms and the launch count of each kernel class
- * [0] column prologue, [1] row pass, [2] column stage 1, [3] column stage 2,
+ * [0] column prologue, [1] row pass, [2] intermediate column stages (IFRK2
+ * stage 1, RK4 stages 1-3), [3] final column stages (IFRK2 stage 2, RK4
+ * stage 4, IFE),
  * then forgets the recorded events. */
 int qs_solver_profile(qs_solver* s, int enable);
 int qs_solver_profile_read(qs_solver* s, double* avg_ms, int* counts);

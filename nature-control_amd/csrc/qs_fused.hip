@@ -192,38 +192,100 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
         st[m] = make_double2(st[m].x + ky * ky * v[m].x, st[m].y + ky * ky * v[m].y);
       }
     }
-    // ---- IFRK2 stage update (state tile read and rewritten in LDS) ---------------
-    tile_load(MODE == MODE_S1 ? a.old_[FLD] : a.new_[FLD]);
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int j = lp + LPC * m;
-      double2* sl = vb + swz(j);
-      const double2 in = *sl;
-      double2 n = st[m];
-      if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) n.y = 0.0;
+    // ---- stage update (state tiles read and rewritten in LDS) ---------------
+    // N_X is in st.  Per element: L_k, IF = exp(L dt), IFh = exp(L dt / 2)
+    // (friction/Qsolver.cuh:20-54 with the D_Q, a_Q coefficients).
+    auto lin = [&](int j) {
       const double ky = kyj(j);
       const double k2 = kx * kx + ky * ky;
-      const double Lk = FLD == 0 ? 1.0 / a.Re * (-1.0 * k2 - a.Rf) : (-1.0 * a.DQ * k2 + a.aQ);
-      const double IF = exp(Lk * a.dt);
-      const double2 an = make_double2(n.x * a.dt, n.y * a.dt);
-      if (MODE == MODE_S1) {
-        const double2 old = in;
-        const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // IFRK2.cuh:16-26
-        const double h = 1.0 / 2.0 * IF;
-        *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // IFRK2.cuh:28-39
-        st[m] = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);      // curr (IFh = IF)
-      } else {
-        const double2 nw = in;
-        double2 r = make_double2(nw.x + 1.0 / 2.0 * an.x, nw.y + 1.0 / 2.0 * an.y);  // IFRK2.cuh:41-51
-        const bool keep = (i < ((a.Nxh - 1) * 2 / 3 + 1)) && ((j < NY / 3 + 1) || ((NY - j) < NY / 3));
-        const double cut = keep ? 1.0 : 0.0;                                  // Qsolver.cuh:314-322
-        r = make_double2(r.x * cut, r.y * cut);
-        if (iok) bad |= !isfinite(r.x) || !isfinite(r.y);
-        *sl = r;
-        st[m] = r;
+      return FLD == 0 ? 1.0 / a.Re * (-1.0 * k2 - a.Rf) : (-1.0 * a.DQ * k2 + a.aQ);
+    };
+    auto keep = [&](int j) {   // dealias mask, Qsolver.cuh:314-322
+      return (i < ((a.Nxh - 1) * 2 / 3 + 1)) && ((j < NY / 3 + 1) || ((NY - j) < NY / 3));
+    };
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {   // real part of N_w at the self-conjugate bins
+      const int j = lp + LPC * m;
+      if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
+    }
+    if (MODE == MODE_S1 || MODE == MODE_S2 || MODE == MODE_R1 || MODE == MODE_R4 || MODE == MODE_E) {
+      // one state tile: S1 / R1 / E read old, S2 / R4 read new
+      const bool from_old = MODE == MODE_S1 || MODE == MODE_R1 || MODE == MODE_E;
+      tile_load(from_old ? a.old_[FLD] : a.new_[FLD]);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int j = lp + LPC * m;
+        double2* sl = vb + swz(j);
+        const double2 in = *sl;
+        const double2 n = st[m];
+        const double Lk = lin(j);
+        const double IF = exp(Lk * a.dt);
+        const double2 an = make_double2(n.x * a.dt, n.y * a.dt);
+        if (MODE == MODE_S1) {
+          const double2 old = in;
+          const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // IFRK2.cuh:16-26
+          const double h = 1.0 / 2.0 * IF;
+          *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // IFRK2.cuh:28-39
+          st[m] = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);      // curr (IFh = IF)
+        } else if (MODE == MODE_R1) {
+          const double2 old = in;
+          const double IFh = exp(Lk * a.dt * 0.5);
+          const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // RK4.cuh:12-22
+          const double h = 1.0 / 6.0 * IF;
+          *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // RK4.cuh:25-36
+          st[m] = make_double2((old.x + an.x / 2.0) * IFh, (old.y + an.y / 2.0) * IFh);
+        } else if (MODE == MODE_E) {
+          const double2 old = in;                                                // IFE.cuh:15-25
+          double2 r = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);
+          const double cut = keep(j) ? 1.0 : 0.0;
+          r = make_double2(r.x * cut, r.y * cut);
+          if (iok) bad |= !isfinite(r.x) || !isfinite(r.y);
+          *sl = r;
+          st[m] = r;
+        } else {
+          const double2 nw = in;
+          double2 r = MODE == MODE_S2
+                          ? make_double2(nw.x + 1.0 / 2.0 * an.x, nw.y + 1.0 / 2.0 * an.y)   // IFRK2.cuh:41-51
+                          : make_double2(nw.x + 1.0 / 6.0 * an.x, nw.y + 1.0 / 6.0 * an.y);  // RK4.cuh:63-74
+          const double cut = keep(j) ? 1.0 : 0.0;
+          r = make_double2(r.x * cut, r.y * cut);
+          if (iok) bad |= !isfinite(r.x) || !isfinite(r.y);
+          *sl = r;
+          st[m] = r;
+        }
+      }
+      // S1, R1 -> new; S2, R4, E -> the next state over old
+      tile_store((MODE == MODE_S1 || MODE == MODE_R1) ? a.new_[FLD] : a.old_[FLD]);
+    } else {
+      // R2, R3: new += 1/3 IFh dt N (new tile), then curr from old (old tile)
+      tile_load(a.new_[FLD]);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int j = lp + LPC * m;
+        double2* sl = vb + swz(j);
+        const double2 nw = *sl;
+        const double IFh = exp(lin(j) * a.dt * 0.5);
+        const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
+        const double h = 1.0 / 3.0 * IFh;                                          // RK4.cuh:37-61
+        *sl = make_double2(nw.x + h * bn.x, nw.y + h * bn.y);
+      }
+      tile_store(a.new_[FLD]);
+      tile_load(a.old_[FLD]);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int j = lp + LPC * m;
+        const double2 old = vb[swz(j)];
+        const double Lk = lin(j);
+        const double IFh = exp(Lk * a.dt * 0.5);
+        const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
+        if (MODE == MODE_R2) {
+          st[m] = make_double2(old.x * IFh + bn.x / 2.0, old.y * IFh + bn.y / 2.0);
+        } else {
+          const double IF = exp(Lk * a.dt);
+          st[m] = make_double2(old.x * IF + bn.x * IFh, old.y * IF + bn.y * IFh);
+        }
       }
     }
-    tile_store(MODE == MODE_S1 ? a.new_[FLD] : a.old_[FLD]);   // stage 2: next state over old
   }
 
   // ---- inverse set of the next stage from st ----------------------------------
@@ -275,7 +337,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     wave_fft<NY, +1>(st, vb, lp, tw);
     stage_out(a.mout[m0 + 1], st);
   }
-  if (MODE == MODE_S2 && bad) atomicOr(a.flag, 1);
+  if ((MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && bad) atomicOr(a.flag, 1);
 }
 
 
@@ -367,9 +429,20 @@ int launch_row_n(const RunDesc* runs, const int2* work, int nwork, int grid, con
 template <int NFIX>
 int launch_col_n(int mode, const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw,
                  hipStream_t st) {
-  if (mode == MODE_PRO) hipLaunchKernelGGL((k_col<NFIX, MODE_PRO>), dim3(grid), dim3(256), 0, st, runs, work, nwork, tw);
-  else if (mode == MODE_S1) hipLaunchKernelGGL((k_col<NFIX, MODE_S1>), dim3(grid), dim3(256), 0, st, runs, work, nwork, tw);
-  else hipLaunchKernelGGL((k_col<NFIX, MODE_S2>), dim3(grid), dim3(256), 0, st, runs, work, nwork, tw);
+  switch (mode) {
+#define QS_COL_MODE(M) \
+    case M: hipLaunchKernelGGL((k_col<NFIX, M>), dim3(grid), dim3(256), 0, st, runs, work, nwork, tw); break;
+    QS_COL_MODE(MODE_PRO)
+    QS_COL_MODE(MODE_S1)
+    QS_COL_MODE(MODE_S2)
+    QS_COL_MODE(MODE_R1)
+    QS_COL_MODE(MODE_R2)
+    QS_COL_MODE(MODE_R3)
+    QS_COL_MODE(MODE_R4)
+    QS_COL_MODE(MODE_E)
+#undef QS_COL_MODE
+    default: return fail(QS_ERR_INVALID_ARG, "column mode %d", mode);
+  }
   QS_CHECK_LAUNCH();
   return QS_OK;
 }

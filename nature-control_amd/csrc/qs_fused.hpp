@@ -5,7 +5,9 @@
 
 namespace qs {
 
-enum { MODE_PRO = 0, MODE_S1 = 1, MODE_S2 = 2 };
+// Column kernel modes: prologue (inverse set of the state), IFRK2 stages 1-2,
+// IF-RK4 stages 1-4, IF-Euler.
+enum { MODE_PRO = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_R1 = 3, MODE_R2 = 4, MODE_R3 = 5, MODE_R4 = 6, MODE_E = 7 };
 // Column kernels of the adjoint step (see qs_adjoint.hip).
 enum { AMODE_START = 0, AMODE_B = 1, AMODE_A = 2, AMODE_END = 3 };
 

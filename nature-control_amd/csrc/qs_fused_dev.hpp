@@ -68,6 +68,7 @@ __device__ __forceinline__ void stage_rows(double2* const* arr, const RunDesc& a
 #pragma unroll
     for (int h = 0; h < 2; ++h) ld[t][h] = gload(arr[t] + off[h]);
   __builtin_amdgcn_sched_barrier(0);
+  __syncthreads();   // the staging area overlaps wave buffers still in use before this call
 #pragma unroll
   for (int t = 0; t < 7; ++t)
 #pragma unroll
@@ -180,7 +181,9 @@ struct ColTile {
     }
     __syncthreads();
   }
-  // cooperative coalesced tile store from the wave buffers
+  // cooperative coalesced tile store from the wave buffers.  Every thread reads
+  // slots of every wave's buffer, so the closing barrier keeps a wave from
+  // rewriting its buffer (its next FFT) while others still read it.
   __device__ __forceinline__ void store(double2* __restrict__ dst) {
     __syncthreads();
 #pragma unroll
@@ -190,6 +193,7 @@ struct ColTile {
       const int ii = i0 + c;
       if (ii < a.Nxh) gstore(dst + (unsigned)(j * a.pitch + ii), sm[slot(j, c)]);
     }
+    __syncthreads();
   }
   __device__ __forceinline__ void stage_in(const double2* __restrict__ src, double2 (&v)[16]) {
     load(src);

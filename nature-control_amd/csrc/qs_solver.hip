@@ -89,7 +89,10 @@ int timed(qs_solver* S, int cls, F launch) {
 
 int row_rhs(qs_solver* S) { return timed(S, K_ROW, [&] { return row_rhs_raw(S); }); }
 int col_stage(qs_solver* S, int mode) {
-  const int cls = mode == MODE_PRO ? K_COL_PRO : (mode == MODE_S1 ? K_COL_S1 : K_COL_S2);
+  // intermediate stages (IFRK2 1, RK4 1-3) and final stages (IFRK2 2, RK4 4, IFE)
+  const int cls = mode == MODE_PRO ? K_COL_PRO
+                  : (mode == MODE_S1 || mode == MODE_R1 || mode == MODE_R2 || mode == MODE_R3) ? K_COL_S1
+                                                                                            : K_COL_S2;
   return timed(S, cls, [&] { return col_stage_raw(S, mode); });
 }
 
@@ -277,6 +280,41 @@ int qs_step_ifrk2(qs_solver* S, int nsteps) {
     QS_TRY(col_stage(S, MODE_S1));
     QS_TRY(row_rhs(S));
     QS_TRY(col_stage(S, MODE_S2));
+  }
+  return QS_OK;
+}
+
+// friction.cu:212-260 verbatim (IF-RK4, RK4.cuh:12-74 with IF = exp(L dt),
+// IFh = exp(L dt / 2)): four right-hand sides per step, each a row pass and a
+// column stage.
+int qs_step_rk4(qs_solver* S, int nsteps) {
+  if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
+  if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
+  for (int n = 0; n < nsteps; ++n) {
+    if (!S->mixed_valid) {
+      QS_TRY(col_stage(S, MODE_PRO));
+      S->mixed_valid = true;
+    }
+    for (int mode : {MODE_R1, MODE_R2, MODE_R3, MODE_R4}) {
+      QS_TRY(row_rhs(S));
+      QS_TRY(col_stage(S, mode));
+    }
+  }
+  return QS_OK;
+}
+
+// IF-Euler (TimeIntegration/IFE.cuh:15-25), then dealias: one right-hand side
+// per step.
+int qs_step_ife(qs_solver* S, int nsteps) {
+  if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
+  if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
+  for (int n = 0; n < nsteps; ++n) {
+    if (!S->mixed_valid) {
+      QS_TRY(col_stage(S, MODE_PRO));
+      S->mixed_valid = true;
+    }
+    QS_TRY(row_rhs(S));
+    QS_TRY(col_stage(S, MODE_E));
   }
   return QS_OK;
 }

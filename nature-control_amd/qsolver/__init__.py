@@ -142,6 +142,8 @@ _SIGS = {
     "qs_debug_wave_fft": [_P, _I, _I, _P, _P],
     "qs_solver_profile_read": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I)],
     "qs_synth_uniform": [ctypes.c_uint64, _D, _D, ctypes.c_size_t, _P],
+    "qs_step_rk4": [_P, _I],
+    "qs_step_ife": [_P, _I],
     "qs_adj_forward": [_P, _I, _I, _P],
     "qs_adj_backward": [_P] * 9,
     "qs_adj_loss": [_P, _I, _P, ctypes.POINTER(_D), _P],
@@ -364,6 +366,12 @@ class Solver:
 
     def step(self, nsteps=1):
         call("qs_step_ifrk2", self._s, int(nsteps))
+
+    def step_rk4(self, nsteps=1):
+        call("qs_step_rk4", self._s, int(nsteps))
+
+    def step_ife(self, nsteps=1):
+        call("qs_step_ife", self._s, int(nsteps))
 
     def step_unfused(self, nsteps=1):
         call("qs_step_ifrk2_unfused", self._s, int(nsteps))

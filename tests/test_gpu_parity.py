@@ -266,3 +266,25 @@ def test_1024_fused_vs_unfused_and_determinism():
     om_cut = host(m.tables()[3])
     for a in states[0]:
         assert np.all(a[om_cut == 0] == 0)
+
+
+@pytest.mark.parametrize("method,N,nsteps,kind", [("rk4", 64, 20, "bandlimited"), ("rk4", 128, 50, "uniform"),
+                                                  ("ife", 64, 30, "bandlimited"), ("ife", 32, 100, "uniform")])
+def test_fused_rk4_and_ife_vs_oracle(method, N, nsteps, kind):
+    """Fused IF-RK4 (the reference driver's integrator, RK4.cuh with IF, IFh)
+    and IF-Euler steps against the oracle's stage kernels."""
+    banner(f"fused {method} N={N} steps={nsteps} init={kind}")
+    st = init_state(N, kind, seed=5)
+    om = O.Mesh(N, N, 2 * np.pi)
+    s = O.Solver(om, oparams(CONFIG, 1e-3, 1))
+    s.set_state(*st)
+    s.step(nsteps, O.RK4 if method == "rk4" else O.IFE)
+    ref = s.get_state()
+    m = Q.Mesh(N, N, 2 * np.pi)
+    g = Q.Solver(m, qparams(CONFIG, 1e-3, 1))
+    g.set_state(*(dev(a) for a in st))
+    (g.step_rk4 if method == "rk4" else g.step_ife)(nsteps)
+    got = [host(t) for t in g.get_state()]
+    assert not g.nonfinite()
+    for a, b in zip(got, ref):
+        assert rel_l2(a, b) < 1e-12
