@@ -146,9 +146,10 @@ int copy_pitched(const qs_solver* S, double2* dst, const double2* src) {
   return QS_OK;
 }
 
-int launch_row(AdjGroup* G, const RunDesc* d) {
+int launch_row(AdjGroup* G, const RunDesc* d) {   // persistent: one block per CU
   const Mesh* m = mesh_of(G);
-  return launch_fused_row(m->Nx, d, G->d_row, G->nrow, G->nrow, m->priv->tw1024, stream_g(G));
+  const int grid = m->priv->ncu > 0 ? std::min(G->nrow, m->priv->ncu) : G->nrow;
+  return launch_fused_row(m->Nx, d, G->d_row, G->nrow, grid, m->priv->tw1024, stream_g(G));
 }
 int launch_col(AdjGroup* G, const RunDesc* d, int mode) {
   const Mesh* m = mesh_of(G);

@@ -91,6 +91,12 @@ int qs_mesh_create(int Nx, int Ny, Qreal Lx, Qreal Ly, int BSZ, Mesh** out) {
   host_twiddles(1024, tw1024.data());
 
   auto cleanup = [&]() { qs_mesh_destroy(m); };
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+      m->priv->ncu = ncu;
+  }
 #define MALLOC(ptr, bytes)                                                   \
   if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess) {                   \
     cleanup();                                                               \

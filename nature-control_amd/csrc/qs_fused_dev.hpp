@@ -92,9 +92,9 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
   const int wave = tid >> 6, lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;
   double2* vb = sm + wave * kBuf + g * VS;
+  auto arr = [&](int t) -> const double2* { return sm + t * RH + g * H; };   // staged array t, vector g
   if (wave < 5) {
     const double w00 = gload(a.w00);
-    const double2* st = sm + g * H;
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
       const int k = lp + LPC * m;
@@ -102,24 +102,24 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
       const double kx = a.dkx * bin;
       double2 X, Y;
       if (wave == 0) {            // (w, dy u + dx v = -W + 2 kx^2 psi + W00 [kx = 0])
-        const double2 W = st[0 * RH + bin], ps = st[1 * RH + bin];
+        const double2 W = arr(0)[bin], ps = arr(1)[bin];
         const double t2 = 2.0 * kx * kx;
         X = W;
         Y = make_double2(-W.x + t2 * ps.x + (bin == 0 ? w00 : 0.0), -W.y + t2 * ps.y);
       } else if (wave == 1) {     // (u = i ky psi, v = -i kx psi)
-        const double2 ps = st[1 * RH + bin];
-        X = st[2 * RH + bin];
+        const double2 ps = arr(1)[bin];
+        X = arr(2)[bin];
         Y = make_double2(kx * ps.y, -kx * ps.x);
       } else if (wave == 2) {     // (r1, r2)
-        X = st[3 * RH + bin];
-        Y = st[5 * RH + bin];
+        X = arr(3)[bin];
+        Y = arr(5)[bin];
       } else if (wave == 3) {     // (L1, L2) = -kx^2 R + R_yy
-        const double2 R1 = st[3 * RH + bin], R1yy = st[4 * RH + bin];
-        const double2 R2 = st[5 * RH + bin], R2yy = st[6 * RH + bin];
+        const double2 R1 = arr(3)[bin], R1yy = arr(4)[bin];
+        const double2 R2 = arr(5)[bin], R2yy = arr(6)[bin];
         X = make_double2(-kx * kx * R1.x + R1yy.x, -kx * kx * R1.y + R1yy.y);
         Y = make_double2(-kx * kx * R2.x + R2yy.x, -kx * kx * R2.y + R2yy.y);
       } else {                    // (dx u = i kx u, -)
-        const double2 U = st[2 * RH + bin];
+        const double2 U = arr(2)[bin];
         X = make_double2(-kx * U.y, kx * U.x);
         Y = make_double2(0.0, 0.0);
       }

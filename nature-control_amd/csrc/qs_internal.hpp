@@ -45,6 +45,7 @@ struct qs_mesh_priv {
   double* kx1 = nullptr;    // kx(i), i < Nxh   (1-D, separable tables)
   double* ky1 = nullptr;    // ky(j), j < Ny
   double2* tw1024 = nullptr; // W_1024^m, m < 1024 (fused kernels use the first 256)
+  int ncu = 0;               // compute units of the device the mesh was created on
 };
 
 namespace qs {

@@ -9,6 +9,7 @@
 // (kernels in qs_fused.hip).  State and mixed arrays live in HBM with a row
 // pitch padded to a multiple of 8 complex values (128-byte aligned rows); the
 // reference layout is used only at qs_solver_set_state / get_state.
+#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -52,8 +53,14 @@ int upload_desc(qs_solver* S) {
 }
 
 int nfix_of(const Mesh* m) { return m->Nx == m->Ny ? m->Nx : 0; }
+// Row pass blocks: one per compute unit (the kernel's LDS admits one block
+// per CU), each looping over its jobs and prefetching the next one's rows.
+int row_grid(const qs_solver* S) {
+  const int ncu = S->mesh->priv->ncu;
+  return ncu > 0 ? std::min(S->nrow, ncu) : S->nrow;
+}
 int row_rhs_raw(qs_solver* S) {
-  return launch_fused_row(nfix_of(S->mesh), S->d_desc, S->d_work, S->nrow, S->nrow, S->mesh->priv->tw1024,
+  return launch_fused_row(nfix_of(S->mesh), S->d_desc, S->d_work, S->nrow, row_grid(S), S->mesh->priv->tw1024,
                           stream_of(S->mesh));
 }
 int col_stage_raw(qs_solver* S, int mode) {
