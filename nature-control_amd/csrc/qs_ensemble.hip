@@ -431,7 +431,7 @@ int qs_ensemble_adj_forward(qs_ensemble* E, int nsteps, int ckpt_every, const Qr
   if (!E->adj) E->adj = adj_group_new(E->runs);
   const int k = ckpt_every <= 0 ? nsteps : std::min(ckpt_every, nsteps);
   QS_TRY(adj_group_forward(E->adj, nsteps, k, alpha_t));
-  E->cls[0].valid = true;   // the mixed arrays hold the final states' inverse sets
+  E->cls[0].valid = false;   // the last segment's inverse sets went to the stage slots
   return QS_OK;
 }
 

@@ -54,7 +54,7 @@ int upload_desc(qs_solver* S) {
 
 int nfix_of(const Mesh* m) { return m->Nx == m->Ny ? m->Nx : 0; }
 // Row pass blocks: one per compute unit (the kernel's LDS admits one block
-// per CU), each looping over its jobs and prefetching the next one's rows.
+// per CU), each looping over its jobs.
 int row_grid(const qs_solver* S) {
   const int ncu = S->mesh->priv->ncu;
   return ncu > 0 ? std::min(S->nrow, ncu) : S->nrow;

@@ -16,9 +16,12 @@ struct AdjState {
   int kcap = 0;          // segment length the stage slots are sized for
   int ncap = 0;          // horizon the per-step arrays are sized for
   int nckpt = 0;         // checkpoints held
+  int nbuf = 0;          // stage-slot buffers (2: a segment is recomputed while another is reversed)
   std::vector<double2*> ckpt;                // 3 per checkpoint, [Ny][pitch]
-  std::vector<double2*> slots;               // 7 per recorded stage, (2 kcap + 1) stages
-  double* w00s = nullptr;                    // W(0,0) per recorded stage
+  std::vector<double2*> slots;               // [buffer][stage][7], (2 kcap + 1) stages per buffer
+  double* w00s = nullptr;                    // W(0,0) per recorded stage, [buffer][2 kcap + 1]
+  double2* amix[7] = {};                     // the reverse sweep's own mixed arrays (nbuf == 2)
+  unsigned long recorded_by = 0;             // id of the group whose last segment the slots still hold
   double2* lam_[3] = {};                     // adjoint state
   double2* acc_[3] = {};
   double* abar = nullptr;                    // [Ny][Nx]

@@ -142,6 +142,33 @@ def test_checkpoint_interval_does_not_change_the_gradient():
         assert np.array_equal(got[1], ref[1]), k
 
 
+@pytest.mark.parametrize("overlap", ["0", "1"])
+def test_overlapped_recompute_and_repeated_backward(monkeypatch, overlap):
+    """Segments recomputed on a second stream beside the reverse sweep
+    (double-buffered stage slots, QS_ADJ_OVERLAP=1) or in turn with it on one
+    stream (QS_ADJ_OVERLAP=0) give the one-segment gradient bit for bit, and a
+    second reverse sweep after the same forward (whose last segment's slots the
+    first sweep may have overwritten) gives it again."""
+    print(f"[gpu-test] adj overlap={overlap}", flush=True)
+    N, dt, n = 64, 1e-3, 11
+    _, X, Ra, _ = case(N, dt, seed=13)
+    monkeypatch.delenv("QS_ADJ_OVERLAP", raising=False)
+    ref = gpu_gradient(N, dt, X, Ra, n, 0, Q.ENSTROPHY)
+    monkeypatch.setenv("QS_ADJ_OVERLAP", overlap)
+    for k in (1, 3, 4):
+        mesh, s = gpu_solver(N, dt)
+        s.set_state(*(cu(x) for x in X))
+        s.set_activity(cu(Ra))
+        s.adj_forward(n, k)
+        L, Wb = Q.adj_loss(mesh, s.get_state()[0], Q.ENSTROPHY)
+        assert L == ref[0]
+        z = torch.zeros_like(Wb)
+        for rep in range(2):
+            ab, _, _ = s.adj_backward(Wb, z, z.clone(), want_field=True)
+            assert np.array_equal(ab.cpu().numpy(), ref[1]), (k, rep)
+        s.close()
+
+
 def test_gradient_vs_finite_differences_on_gpu():
     """Central differences of the GPU forward along random activity fields."""
     print("[gpu-test] adj finite differences", flush=True)
