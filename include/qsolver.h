@@ -44,7 +44,8 @@ typedef enum qs_status {
   QS_ERR_HIP = 3,
   QS_ERR_NONFINITE = 4,
   QS_ERR_OOM = 5,
-  QS_ERR_STATE = 6
+  QS_ERR_STATE = 6,
+  QS_ERR_IO = 7
 } qs_status;
 
 typedef struct qs_mesh_priv qs_mesh_priv;
@@ -236,9 +237,12 @@ int qs_debug_wave_fft(const Mesh* mesh, int n, int sign, const Qcomp* in, Qcomp*
 int qs_adj_forward(qs_solver* s, int nsteps, int ckpt_every, const Qreal* alpha_t);
 /* Reverse sweep of the last qs_adj_forward.  gW, gR1, gR2: dL/d(final state).
  * Outputs (each may be NULL): abar = dL/dRa [Ny*Nx]; abar_t = dL/dalpha_t
- * [nsteps]; uW, uR1, uR2 = dL/d(initial state).  Every segment is recomputed
- * from its checkpoint, recording each stage's inverse set (7 spectra per
- * stage), then reversed.  Leaves the solver at the final state. */
+ * [nsteps]; uW, uR1, uR2 = dL/d(initial state).  Segments are reversed last
+ * to first, each with every stage's inverse set (7 spectra per stage) at hand:
+ * qs_adj_forward records the last segment's, the others are recomputed from
+ * their checkpoints (beside the reverse sweep, on a second stream, when one
+ * launch leaves compute units idle; QS_ADJ_OVERLAP=0|1 in the environment
+ * forces the choice).  Leaves the solver at the final state. */
 int qs_adj_backward(qs_solver* s, const Qcomp* gW, const Qcomp* gR1, const Qcomp* gR2,
                     Qreal* abar, Qreal* abar_t, Qcomp* uW, Qcomp* uR1, Qcomp* uR2);
 /* Loss of a vorticity spectrum W and dL/dW: kind 0 = 1/2 mean(omega^2) with
@@ -296,6 +300,48 @@ int qs_ensemble_adj_loss(qs_ensemble* e, int kind, Qreal* losses);
  * abar_t [n][nsteps] = dL/dalpha_t, abar [n][Ny*Nx] = dL/dRa. */
 int qs_ensemble_adj_backward(qs_ensemble* e, Qreal* abar_t, Qreal* abar);
 int qs_ensemble_adj_memory(qs_ensemble* e, size_t* bytes);
+
+/* ---- diagnostics (device reductions; results in device memory) ----------- */
+/* Etot_func (InertialFriction/post_process/postprocess.cu:54-65):
+ * *E = sum (u^2 + v^2) * 0.5 * dx * dy over the physical arrays u, v; the
+ * driver divides by Lx*Ly (:161).  Deterministic (fixed summation order). */
+int qs_Etot_func(const Mesh* mesh, const Qreal* u, const Qreal* v, Qreal* E);
+/* Espec_func (post_process/EnergySpec.cu:81-96): *E = sum of |ut|^2 + |vt|^2
+ * over the half-spectrum bins with k^2 <= kx^2 + ky^2 < (k + alpha)^2, divided
+ * by 2 alpha Nx^2 Ny^2.  ut, vt: velocity spectra (qs_vel_func). */
+int qs_Espec_func(const Mesh* mesh, Qreal k, const Qcomp* ut, const Qcomp* vt, Qreal alpha, Qreal* E);
+/* Every shell of the driver loop EnergySpec.cu:198-204 (k = 0, dk, 2 dk, ...
+ * by repeated addition while k < kmax) in one pass: E[0..*nshell) (device),
+ * QS_ERR_INVALID_ARG when there are more than nmax shells. */
+int qs_Espec_shells(const Mesh* mesh, const Qcomp* ut, const Qcomp* vt, Qreal dk, Qreal kmax, int nmax,
+                    Qreal* E, int* nshell);
+
+/* ---- snapshot and restart I/O -------------------------------------------
+ * CSV text in the reference's format (C++ stream defaults, 6 significant
+ * digits, one comma-terminated line per row j).  f arrays are HOST memory
+ * here, as in the reference. */
+int qs_field_visual(const Qreal* f, const char* name, int Nx, int Ny);   /* BasicUtils.cuh:38-50 */
+int qs_wave_visual(const Qreal* f, const char* name, int Nxh, int Ny);   /* EnergySpec.cu:8-20 */
+int qs_spec_visual(const Qcomp* f, const char* name, int Nxh, int Ny);   /* EnergySpec.cu:22-34 */
+/* BasicUtils.cuh:52-67 (x = dx i, y = dy j as float); NULL names = "x.csv", "y.csv". */
+int qs_coord(Qreal dx, Qreal dy, int Nx, int Ny, const char* xname, const char* yname);
+/* friction.cu:62-97.  A missing file or a row count / row length other than
+ * the mesh's is an error (the reference exits or returns leaving f as it was). */
+int qs_file_init(const char* filename, Qreal* f, const Mesh* mesh);
+/* friction.cu:262-279: <prefix><step>w.csv, r1.csv, r2.csv (BwdTrans of the
+ * state, whose column-0 symmetrisation lands in the state as there) and
+ * S.csv; QS_ERR_NONFINITE when S[0] is NaN. */
+int qs_solver_snapshot_csv(qs_solver* s, const char* prefix, int step);
+/* friction.cu:163-185: state from three physical CSVs (file_init, FwdTrans,
+ * the BwdTrans round trip that symmetrises column 0). */
+int qs_solver_init_csv(qs_solver* s, const char* w_csv, const char* r1_csv, const char* r2_csv);
+/* No reference counterpart (the reference restarts from its 6-digit CSVs):
+ * exact binary checkpoint of the state spectra and the activity field, with
+ * the mesh, the parameters, a caller's step counter and a checksum.  load
+ * requires the same mesh; it restores state and activity and returns the
+ * stored step and parameters (each may be NULL) without applying the latter. */
+int qs_solver_save(qs_solver* s, const char* path, int64_t step);
+int qs_solver_load(qs_solver* s, const char* path, int64_t* step, qs_params* params);
 
 /* ---- host helpers (no GPU) ---------------------------------------------- */
 /* splitmix64 uniform [0,1) stream: u_k = (x_k >> 11) * 2^-53. */

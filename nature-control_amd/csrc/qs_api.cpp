@@ -52,6 +52,7 @@ const char* qs_status_string(int s) {
     case QS_ERR_NONFINITE: return "non-finite value in state";
     case QS_ERR_OOM: return "out of device memory";
     case QS_ERR_STATE: return "invalid object state";
+    case QS_ERR_IO: return "file I/O error";
     default: return "unknown status";
   }
 }
@@ -137,7 +138,8 @@ int qs_mesh_destroy(Mesh* m) {
   void* ptrs[] = {m->spec, m->phys, m->kx, m->ky, m->k_squared, m->cutoff,
                   m->priv ? m->priv->twx : nullptr, m->priv ? m->priv->twy : nullptr,
                   m->priv ? m->priv->kx1 : nullptr, m->priv ? m->priv->ky1 : nullptr,
-                  m->priv ? (void*)m->priv->tw1024 : nullptr};
+                  m->priv ? (void*)m->priv->tw1024 : nullptr, m->priv ? (void*)m->priv->diag : nullptr};
+  if (m->priv && m->priv->diag) (void)hipStreamSynchronize(m->priv->stream);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m->priv;

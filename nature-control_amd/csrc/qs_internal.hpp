@@ -46,6 +46,8 @@ struct qs_mesh_priv {
   double* ky1 = nullptr;    // ky(j), j < Ny
   double2* tw1024 = nullptr; // W_1024^m, m < 1024 (fused kernels use the first 256)
   int ncu = 0;               // compute units of the device the mesh was created on
+  double* diag = nullptr;    // partial sums of the diagnostics (qs_diag.hip), grown on demand
+  size_t diag_n = 0;
 };
 
 namespace qs {

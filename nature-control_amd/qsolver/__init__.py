@@ -20,8 +20,11 @@ _D = ctypes.c_double
 _I = ctypes.c_int
 
 QS_OK = 0
+(QS_ERR_INVALID_ARG, QS_ERR_UNSUPPORTED_SIZE, QS_ERR_HIP, QS_ERR_NONFINITE, QS_ERR_OOM, QS_ERR_STATE,
+ QS_ERR_IO) = range(1, 8)
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported size", 3: "HIP runtime error",
-          4: "non-finite value in state", 5: "out of device memory", 6: "invalid object state"}
+          4: "non-finite value in state", 5: "out of device memory", 6: "invalid object state",
+          7: "file I/O error"}
 
 
 class QsolverError(RuntimeError):
@@ -165,6 +168,18 @@ _SIGS = {
     "qs_ensemble_adj_loss": [_P, _I, _P],
     "qs_ensemble_adj_backward": [_P, _P, _P],
     "qs_ensemble_adj_memory": [_P, ctypes.POINTER(ctypes.c_size_t)],
+    "qs_Etot_func": [_P, _P, _P, _P],
+    "qs_Espec_func": [_P, _D, _P, _P, _D, _P],
+    "qs_Espec_shells": [_P, _P, _P, _D, _D, _I, _P, ctypes.POINTER(_I)],
+    "qs_field_visual": [_P, ctypes.c_char_p, _I, _I],
+    "qs_wave_visual": [_P, ctypes.c_char_p, _I, _I],
+    "qs_spec_visual": [_P, ctypes.c_char_p, _I, _I],
+    "qs_coord": [_D, _D, _I, _I, ctypes.c_char_p, ctypes.c_char_p],
+    "qs_file_init": [ctypes.c_char_p, _P, _P],
+    "qs_solver_snapshot_csv": [_P, ctypes.c_char_p, _I],
+    "qs_solver_init_csv": [_P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p],
+    "qs_solver_save": [_P, ctypes.c_char_p, ctypes.c_int64],
+    "qs_solver_load": [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(QsParams)],
 }
 _RESTYPE = {"qs_last_error": ctypes.c_char_p, "qs_status_string": ctypes.c_char_p}
 
@@ -290,6 +305,36 @@ class Mesh:
         conv = [ptr(a) if isinstance(a, torch.Tensor) else a for a in args]
         call("qs_" + name, *conv, self._m)
 
+    # ---- diagnostics (post_process/postprocess.cu, EnergySpec.cu) ---------
+    def Etot(self, u, v):
+        """Etot_func of physical velocity fields: sum (u^2+v^2) dx dy / 2
+        (divide by Lx*Ly for the driver's Etot)."""
+        E = torch.zeros(1, dtype=torch.float64, device="cuda")
+        call("qs_Etot_func", self._m, ptr(u), ptr(v), ptr(E))
+        return float(E.item())
+
+    def Espec(self, k, ut, vt, alpha):
+        """Espec_func of one shell k^2 <= |K|^2 < (k+alpha)^2."""
+        E = torch.zeros(1, dtype=torch.float64, device="cuda")
+        call("qs_Espec_func", self._m, float(k), ptr(ut), ptr(vt), float(alpha), ptr(E))
+        return float(E.item())
+
+    def Espec_shells(self, ut, vt, dk, kmax, nmax=1 << 16):
+        """Every shell of the EnergySpec driver loop (k = 0, dk, ... < kmax)
+        as a cuda float64 tensor."""
+        E = torch.zeros(nmax, dtype=torch.float64, device="cuda")
+        n = _I(0)
+        call("qs_Espec_shells", self._m, ptr(ut), ptr(vt), float(dk), float(kmax), int(nmax), ptr(E),
+             ctypes.byref(n))
+        return E[: n.value].clone()
+
+    def file_init(self, path):
+        """file_init (friction.cu:62-97): a physical CSV as a float64 [Ny, Nx] numpy array."""
+        import numpy as np
+        out = np.empty(self.phys_shape, dtype=np.float64)
+        call("qs_file_init", os.fsencode(path), out.ctypes.data_as(_P), self._m)
+        return out
+
 
 class Field:
     """Mirror of the reference Field (Basic/Mesh.cuh:115-129)."""
@@ -401,6 +446,25 @@ class Solver:
         call("qs_solver_nonfinite", self._s, ctypes.byref(flag))
         return bool(flag.value)
 
+    # ---- snapshot and restart (friction.cu:163-185, :262-279) --------------
+    def snapshot_csv(self, prefix, step):
+        """<prefix><step>{w,r1,r2,S}.csv of the current state."""
+        call("qs_solver_snapshot_csv", self._s, os.fsencode(str(prefix)), int(step))
+
+    def init_csv(self, w_csv, r1_csv, r2_csv):
+        call("qs_solver_init_csv", self._s, os.fsencode(w_csv), os.fsencode(r1_csv), os.fsencode(r2_csv))
+
+    def save(self, path, step=0):
+        """Exact binary checkpoint of state and activity."""
+        call("qs_solver_save", self._s, os.fsencode(path), int(step))
+
+    def load(self, path):
+        """Restores state and activity; returns (stored step, stored qs_params)."""
+        st = ctypes.c_int64(0)
+        p = QsParams()
+        call("qs_solver_load", self._s, os.fsencode(path), ctypes.byref(st), ctypes.byref(p))
+        return st.value, p
+
     # ---- discrete adjoint (qs_adj_*) ---------------------------------------
     def adj_forward(self, nsteps, ckpt_every=0, alpha_t=None):
         """nsteps steps from the current state with checkpoints every
@@ -427,6 +491,38 @@ class Solver:
 
 
 ENSTROPHY, KINETIC = 0, 1
+
+
+def _host(a, dtype):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data_as(_P)
+
+
+def field_visual(f, name):
+    """field_visual (BasicUtils.cuh:38-50) of a host [Ny, Nx] float64 array."""
+    import numpy as np
+    a, p = _host(f, np.float64)
+    call("qs_field_visual", p, os.fsencode(name), a.shape[1], a.shape[0])
+
+
+def wave_visual(f, name):
+    """wave_visual (EnergySpec.cu:8-20) of a host [Ny, Nxh] float64 array."""
+    import numpy as np
+    a, p = _host(f, np.float64)
+    call("qs_wave_visual", p, os.fsencode(name), a.shape[1], a.shape[0])
+
+
+def spec_visual(f, name):
+    """spec_visual (EnergySpec.cu:22-34) of a host [Ny, Nxh] complex128 array."""
+    import numpy as np
+    a, p = _host(f, np.complex128)
+    call("qs_spec_visual", p, os.fsencode(name), a.shape[1], a.shape[0])
+
+
+def coord(dx, dy, Nx, Ny, xname="x.csv", yname="y.csv"):
+    """coord (BasicUtils.cuh:52-67)."""
+    call("qs_coord", float(dx), float(dy), int(Nx), int(Ny), os.fsencode(xname), os.fsencode(yname))
 
 
 def adj_loss(mesh, W, kind=ENSTROPHY):

@@ -71,6 +71,9 @@ def lib():
             "or_adj_step_vjp": (None, [_P] * 11),
             "or_adj_loss_grad": (_D, [_P, _P, _I, _P]),
             "or_adj_gradient": (_D, [_P, _I, _I, _P, _P, _P]),
+            "or_Etot_func": (_D, [_P, _P, _I, _I, _D, _D]),
+            "or_Espec_func": (_D, [_P, _D, _P, _P, _D]),
+            "or_Espec_shells": (_I, [_P, _P, _P, _D, _D, _I, _P]),
             "or_set_threads": (None, [_I]),
             "or_get_threads": (_I, []),
         }
@@ -144,6 +147,34 @@ class Mesh:
         f = np.empty(self.phys_shape)
         lib().or_BwdTrans(_ptr(f), _ptr(ft), self.h)
         return f
+
+    def vel(self, wt):
+        """vel_func (FldOp/QFldfuncs.cuh:204-224): (u, v) spectra of a vorticity spectrum."""
+        k2 = self.k_squared.copy()
+        k2[0, 0] = 1.0
+        psi = wt / k2
+        psi[0, 0] = 0.0
+        return 1j * self.ky * psi, -1j * self.kx * psi
+
+    def Etot(self, u, v):
+        """Etot_func (post_process/postprocess.cu:54-65)."""
+        u = np.ascontiguousarray(u, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        return lib().or_Etot_func(_ptr(u), _ptr(v), self.Nx, self.Ny, self.Lx / self.Nx, self.Ly / self.Ny)
+
+    def Espec(self, k, ut, vt, alpha):
+        """Espec_func (post_process/EnergySpec.cu:81-96), one shell."""
+        ut = np.ascontiguousarray(ut, np.complex128)
+        vt = np.ascontiguousarray(vt, np.complex128)
+        return lib().or_Espec_func(self.h, k, _ptr(ut), _ptr(vt), alpha)
+
+    def Espec_shells(self, ut, vt, dk, kmax, nmax=4096):
+        """The EnergySpec.cu:194-204 driver loop: one value per shell."""
+        ut = np.ascontiguousarray(ut, np.complex128)
+        vt = np.ascontiguousarray(vt, np.complex128)
+        E = np.empty(nmax)
+        n = lib().or_Espec_shells(self.h, _ptr(ut), _ptr(vt), dk, kmax, nmax, _ptr(E))
+        return E[:n].copy()
 
 
 class Field:

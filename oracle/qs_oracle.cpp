@@ -1133,6 +1133,45 @@ double or_adj_gradient(void* sv, int nsteps, int kind, const double* alpha_t, do
   return L;
 }
 
+// InertialFriction/post_process/postprocess.cu:54-65, same summation order;
+// products rounded one by one (no fused multiply-add), as on the host.
+__attribute__((optimize("fp-contract=off"))) double or_Etot_func(const double* u, const double* v, int Nx,
+                                                                  int Ny, double dx, double dy) {
+  double E = 0.0;
+  for (int j = 0; j < Ny; j++)
+    for (int i = 0; i < Nx; i++) {
+      const int index = j * Nx + i;
+      E += (u[index] * u[index] + v[index] * v[index]) * 0.5 * dx * dy;
+    }
+  return E;
+}
+
+// post_process/EnergySpec.cu:81-96 over the mesh's wave tables.
+__attribute__((optimize("fp-contract=off"))) double or_Espec_func(void* mv, double k, const or_comp* ut,
+                                                                   const or_comp* vt, double alpha) {
+  const OMesh* m = static_cast<OMesh*>(mv);
+  const int Nxh = m->Nxh, Ny = m->Ny;
+  double E = 0.0;
+  for (int j = 0; j < Ny; j++)
+    for (int i = 0; i < Nxh; i++) {
+      const int index = j * Nxh + i;
+      const double mod = m->kx[index] * m->kx[index] + m->ky[index] * m->ky[index];
+      if ((mod < ((k + alpha) * (k + alpha))) && (mod >= (k * k)))
+        E += ut[index].x * ut[index].x + ut[index].y * ut[index].y + vt[index].x * vt[index].x +
+             vt[index].y * vt[index].y;
+    }
+  return E / (2.0 * alpha * (Nxh - 1) * 2 * Ny * (Nxh - 1) * 2 * Ny);
+}
+
+// The driver loop of EnergySpec.cu:194-204: k = 0, dk, ... while k < kmax.
+// Returns the number of shells written to E (at most nmax).
+int or_Espec_shells(void* mv, const or_comp* ut, const or_comp* vt, double dk, double kmax, int nmax,
+                    double* E) {
+  int n = 0;
+  for (double k = 0.0; k < kmax && n < nmax; k += dk) E[n++] = or_Espec_func(mv, k, ut, vt, dk);
+  return n;
+}
+
 void or_set_threads(int n) { g_threads = n; }
 int or_get_threads(void) { return nthreads(); }
 

@@ -116,6 +116,13 @@ double or_adj_gradient(void* s, int nsteps, int kind, const double* alpha_t,
                        double* abar_field, double* abar_t);
 
 /* Threads used by the oracle's OpenMP loops (0 = library default). */
+/* Diagnostics: Etot_func (post_process/postprocess.cu:54-65) and Espec_func
+ * (post_process/EnergySpec.cu:81-96, one shell; shells: its driver loop). */
+double or_Etot_func(const double* u, const double* v, int Nx, int Ny, double dx, double dy);
+double or_Espec_func(void* mesh, double k, const or_comp* ut, const or_comp* vt, double alpha);
+int    or_Espec_shells(void* mesh, const or_comp* ut, const or_comp* vt, double dk, double kmax, int nmax,
+                       double* E);
+
 void  or_set_threads(int n);
 int   or_get_threads(void);
 
