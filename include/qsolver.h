@@ -22,7 +22,9 @@
  *    derivative ops, e.g. friction/Qsolver.cuh:181,238).  FwdTrans/BwdTrans
  *    need f != ft and preserve their input, except BwdTrans's documented
  *    column-0 symmetrisation of ft (FldOp/QFldfuncs.cuh:50-55).
- *  - Supported meshes: Nx, Ny powers of two in [16, 2048].
+ *  - Supported meshes: Nx, Ny = 2^a in [16, 2048] or 3 * 2^a in [48, 1536]
+ *    (the per-op calls and the unfused steps); the fused solver, its adjoint
+ *    and ensembles take powers of two up to 1024.
  */
 #ifndef QSOLVER_H
 #define QSOLVER_H
@@ -200,8 +202,13 @@ int qs_step_rk4(qs_solver* s, int nsteps);
  * into one row pass and one column stage per step. */
 int qs_step_ife(qs_solver* s, int nsteps);
 /* Same step built from the per-op entry points above (reference call
- * granularity); for parity and for timing the unfused structure. */
+ * granularity); for parity and for timing the unfused structure.  Runs on
+ * every supported mesh, including 3 * 2^a sides such as the reference
+ * driver's 768 (the fused steps take powers of two up to 1024). */
 int qs_step_ifrk2_unfused(qs_solver* s, int nsteps);
+/* The reference driver's RK4 step loop itself (friction.cu:212-260 with
+ * TimeIntegration/RK4.cuh:12-74) through the per-op entry points. */
+int qs_step_rk4_unfused(qs_solver* s, int nsteps);
 /* 1 if a non-finite value reached the state since the last query. */
 int qs_solver_nonfinite(qs_solver* s, int* flag);
 /* No reference counterpart (the reference has no timers).  While enabled,

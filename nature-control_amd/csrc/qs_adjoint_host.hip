@@ -428,6 +428,7 @@ int group_backward(AdjGroup* G) {
 
 int square_check(const Mesh* m) {
   if (m->Nx != m->Ny) return fail(QS_ERR_UNSUPPORTED_SIZE, "adjoint: square meshes only");
+  if (!fused_side(m->Nx)) return fail(QS_ERR_UNSUPPORTED_SIZE, "adjoint: mesh %d (powers of two, 16..1024)", m->Nx);
   return QS_OK;
 }
 

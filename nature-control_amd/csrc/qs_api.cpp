@@ -61,9 +61,9 @@ const char* qs_status_string(int s) {
 int qs_mesh_create(int Nx, int Ny, Qreal Lx, Qreal Ly, int BSZ, Mesh** out) {
   if (!out) return fail(QS_ERR_INVALID_ARG, "qs_mesh_create: null out");
   *out = nullptr;
-  if (!is_pow2(Nx) || !is_pow2(Ny) || Nx < 16 || Ny < 16 || Nx > 2048 || Ny > 2048)
-    return fail(QS_ERR_UNSUPPORTED_SIZE, "qs_mesh_create: Nx=%d Ny=%d (powers of two in [16,2048])",
-                Nx, Ny);
+  if (!supported_side(Nx) || !supported_side(Ny))
+    return fail(QS_ERR_UNSUPPORTED_SIZE,
+                "qs_mesh_create: Nx=%d Ny=%d (2^a in [16,2048] or 3*2^a in [48,1536])", Nx, Ny);
   if (!(Lx > 0) || !(Ly > 0)) return fail(QS_ERR_INVALID_ARG, "qs_mesh_create: Lx, Ly must be > 0");
   Mesh* m = (Mesh*)std::calloc(1, sizeof(Mesh));
   if (!m) return fail(QS_ERR_OOM, "host allocation");

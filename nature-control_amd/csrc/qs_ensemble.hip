@@ -230,6 +230,8 @@ int qs_ensemble_create(int n, const int* N, const qs_params* params, qs_ensemble
   std::sort(sizes.begin(), sizes.end());
   sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
   if (sizes.size() > 8) return bail(fail(QS_ERR_INVALID_ARG, "qs_ensemble_create: more than 8 mesh sizes"));
+  for (int s : sizes)
+    if (!fused_side(s)) return bail(fail(QS_ERR_UNSUPPORTED_SIZE, "qs_ensemble_create: mesh %d (powers of two, 16..1024)", s));
   for (int s : sizes) {
     EnsClass c;
     c.N = s;

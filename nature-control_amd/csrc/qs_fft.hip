@@ -1,7 +1,10 @@
 // qs_fft.hip -- per-op 2-D real transforms in the reference layouts.
 //
 // Replaces the cuFFT plans of Basic/Mesh.cuh:72-73 as used by FwdTrans /
-// BwdTrans (FldOp/QFldfuncs.cuh:45-55).  A 2-D transform is two kernels:
+// BwdTrans (FldOp/QFldfuncs.cuh:45-55) for Nx, Ny = 2^a (16..2048) and
+// 3 * 2^a (48..1536, the reference driver's 768 = 512 * 3/2, friction.cu:109).
+// Stockham passes of radix 4, then 2 and a last radix-3 pass where needed.
+// A 2-D transform is two kernels:
 //   R2C: row pass  (x-direction real FFT, two rows per complex FFT)  -> mixed
 //        column pass (y-direction complex FFT on LDS tiles of columns) -> ft
 //   C2R: column pass (inverse; also writes BwdTrans's column-0 symmetrisation
@@ -55,7 +58,7 @@ __global__ __launch_bounds__(RowCfg<N>::NT) void k_row_r2c(const double* __restr
     const int vec = e / H, k = e % H;
     const int y0 = 2 * (pair0 + vec), y1 = y0 + 1;
     const double2 zk = s[vec * N + k];
-    const double2 zc = cconj(s[vec * N + ((N - k) & (N - 1))]);
+    const double2 zc = cconj(s[vec * N + (N - k) % N]);
     const double2 A = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
     const double2 d = csub(zk, zc);
     const double2 B = make_double2(0.5 * d.y, -0.5 * d.x);
@@ -163,8 +166,15 @@ int launch_col(const Mesh* m, const double2* in, double2* out, double2* sym) {
   return QS_OK;
 }
 
-#define QS_DISPATCH_POW2(n, FN, ...)                                   \
+// Powers of two 16..2048 and 3 * 2^a 48..1536 (radix-3 last pass).
+#define QS_DISPATCH_N(n, FN, ...)                                   \
   switch (n) {                                                          \
+    case 48: return FN<48>(__VA_ARGS__);                                \
+    case 96: return FN<96>(__VA_ARGS__);                                \
+    case 192: return FN<192>(__VA_ARGS__);                              \
+    case 384: return FN<384>(__VA_ARGS__);                              \
+    case 768: return FN<768>(__VA_ARGS__);                              \
+    case 1536: return FN<1536>(__VA_ARGS__);                            \
     case 16: return FN<16>(__VA_ARGS__);                                \
     case 32: return FN<32>(__VA_ARGS__);                                \
     case 64: return FN<64>(__VA_ARGS__);                                \
@@ -183,11 +193,11 @@ template <int NY> int col_inv(const Mesh* m, const double2* in, double2* out, do
   return sym ? launch_col<NY, +1, true>(m, in, out, sym) : launch_col<NY, +1, false>(m, in, out, nullptr);
 }
 
-int row_r2c(const Mesh* m, const double* f, double2* mix) { QS_DISPATCH_POW2(m->Nx, launch_row_r2c, m, f, mix); }
-int row_c2r(const Mesh* m, const double2* mix, double* f) { QS_DISPATCH_POW2(m->Nx, launch_row_c2r, m, mix, f); }
-int colf(const Mesh* m, const double2* in, double2* out) { QS_DISPATCH_POW2(m->Ny, col_fwd, m, in, out); }
+int row_r2c(const Mesh* m, const double* f, double2* mix) { QS_DISPATCH_N(m->Nx, launch_row_r2c, m, f, mix); }
+int row_c2r(const Mesh* m, const double2* mix, double* f) { QS_DISPATCH_N(m->Nx, launch_row_c2r, m, mix, f); }
+int colf(const Mesh* m, const double2* in, double2* out) { QS_DISPATCH_N(m->Ny, col_fwd, m, in, out); }
 int coli(const Mesh* m, const double2* in, double2* out, double2* sym) {
-  QS_DISPATCH_POW2(m->Ny, col_inv, m, in, out, sym);
+  QS_DISPATCH_N(m->Ny, col_inv, m, in, out, sym);
 }
 
 }  // namespace

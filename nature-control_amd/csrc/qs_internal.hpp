@@ -33,6 +33,12 @@ int fail(int status, const char* fmt, ...);
   } while (0)
 
 inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+// Mesh sides the per-op transforms support: 2^a in [16, 2048] and 3 * 2^a in
+// [48, 1536].  The fused kernels take the powers of two up to 1024.
+inline bool supported_side(int n) {
+  return (is_pow2(n) && n >= 16 && n <= 2048) || (n % 3 == 0 && is_pow2(n / 3) && n >= 48 && n <= 1536);
+}
+inline bool fused_side(int n) { return is_pow2(n) && n >= 16 && n <= 1024; }
 inline int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
 
 }  // namespace qs
@@ -147,11 +153,34 @@ __device__ __forceinline__ void dft8(double2* v) {
   }
 }
 
+// radix-3: y0 = a + b + c, y1,2 = a - (b + c)/2 -/+ SIGN' i (sqrt(3)/2)(b - c)
+// with W3 = exp(SIGN 2 pi i / 3).
+template <int SIGN>
+__device__ __forceinline__ void dft3(double2* v) {
+  constexpr double S3 = 0.86602540378443864676;  // sin(pi/3)
+  const double2 s = cadd(v[1], v[2]);
+  const double2 t = make_double2(v[0].x - 0.5 * s.x, v[0].y - 0.5 * s.y);
+  const double2 d = make_double2(S3 * (v[1].x - v[2].x), S3 * (v[1].y - v[2].y));
+  const double2 id = mul_pmi<SIGN>(d);   // SIGN i d
+  v[0] = cadd(v[0], s);
+  v[1] = cadd(t, id);
+  v[2] = csub(t, id);
+}
+
 template <int R, int SIGN>
 __device__ __forceinline__ void dft_small(double2* v) {
   if constexpr (R == 2) dft2<SIGN>(v[0], v[1]);
+  else if constexpr (R == 3) dft3<SIGN>(v);
   else if constexpr (R == 4) dft4<SIGN>(v);
   else if constexpr (R == 8) dft8<SIGN>(v);
+  else static_assert(R == 2 || R == 3 || R == 4 || R == 8, "radix");
+}
+
+// Radix of the Stockham pass that meets M = N / P remaining points: RMAX
+// while it divides M, then a power of two below RMAX, then 4, 2, and finally
+// 3 (N = 2^a or 3 * 2^a).
+constexpr int stockham_radix(int M, int RMAX) {
+  return M % RMAX == 0 ? RMAX : ((M & (M - 1)) == 0 ? M : (M % 4 == 0 ? 4 : (M % 2 == 0 ? 2 : 3)));
 }
 
 // ---------------------------------------------------------------------------
@@ -240,7 +269,7 @@ __device__ __forceinline__ void stockham_pass(double2* s, const TW& tw, int tid)
 template <int N, int RMAX, int P, int SIGN, int NV, int LD, int NT, class TW>
 __device__ __forceinline__ void stockham_from(double2* s, const TW& tw, int tid) {
   if constexpr (P < N) {
-    constexpr int R = (N / P >= RMAX) ? RMAX : (N / P);
+    constexpr int R = stockham_radix(N / P, RMAX);
     stockham_pass<N, R, P, SIGN, NV, LD, NT>(s, tw, tid);
     stockham_from<N, RMAX, P * R, SIGN, NV, LD, NT>(s, tw, tid);
   }

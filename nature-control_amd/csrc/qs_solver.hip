@@ -59,11 +59,22 @@ int row_grid(const qs_solver* S) {
   const int ncu = S->mesh->priv->ncu;
   return ncu > 0 ? std::min(S->nrow, ncu) : S->nrow;
 }
+// The fused kernels are instantiated for power-of-two sides up to 1024; other
+// meshes (3 * 2^a, 2048) run the per-op path (qs_step_ifrk2_unfused).
+int fused_check(const qs_solver* S) {
+  const Mesh* m = S->mesh;
+  if (!fused_side(m->Nx) || !fused_side(m->Ny))
+    return fail(QS_ERR_UNSUPPORTED_SIZE, "fused step: %dx%d mesh (powers of two up to 1024; qs_step_ifrk2_unfused "
+                "runs any supported mesh)", m->Nx, m->Ny);
+  return QS_OK;
+}
 int row_rhs_raw(qs_solver* S) {
+  QS_TRY(fused_check(S));
   return launch_fused_row(nfix_of(S->mesh), S->d_desc, S->d_work, S->nrow, row_grid(S), S->mesh->priv->tw1024,
                           stream_of(S->mesh));
 }
 int col_stage_raw(qs_solver* S, int mode) {
+  QS_TRY(fused_check(S));
   return launch_fused_col(nfix_of(S->mesh), mode, S->d_desc, S->d_work, S->ncol, S->ncol,
                           S->mesh->priv->tw1024, stream_of(S->mesh));
 }
@@ -351,6 +362,45 @@ int qs_step_ifrk2_unfused(qs_solver* S, int nsteps) {
     QS_TRY(unfused_rhs(S));
     for (int k = 0; k < 3; ++k)
       QS_TRY(qs_ifrk2_func2D(S->uold[k], curr[k]->spec, S->unew[k], nonl[k]->spec, IF[k], IF[k], dt, m));
+    for (int k = 0; k < 3; ++k) QS_TRY(qs_dealias(S->unew[k], m));
+    for (int k = 0; k < 3; ++k) QS_TRY(qs_SpecSet(S->uold[k], S->unew[k], m));
+  }
+  for (int k = 0; k < 3; ++k)
+    QS_HIP(hipMemcpy2DAsync(S->old_[k], prow, S->uold[k], row, row, m->Ny, hipMemcpyDeviceToDevice,
+                            stream_of(m)));
+  S->mixed_valid = false;
+  return QS_OK;
+}
+
+// The reference driver's own step (friction.cu:212-260 with RK4.cuh:12-74:
+// IF = exp(L dt), IFh = exp(L dt/2)) through the per-op entry points.
+int qs_step_rk4_unfused(qs_solver* S, int nsteps) {
+  if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
+  if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
+  QS_TRY(unfused_setup(S));
+  QS_TRY(unfused_activity(S));
+  Mesh* m = S->mesh;
+  Field** F = S->F;
+  const double dt = S->p.dt;
+  Field* curr[3] = {F[F_WCURR], F[F_R1CURR], F[F_R2CURR]};
+  Field* nonl[3] = {F[F_WNONL], F[F_R1NONL], F[F_R2NONL]};
+  const double* IF[3] = {S->IFt[0], S->IFt[2], S->IFt[2]};
+  const double* IFh[3] = {S->IFt[1], S->IFt[3], S->IFt[3]};
+  const size_t row = sizeof(Qcomp) * m->Nxh;
+  const size_t prow = sizeof(double2) * S->pitch;
+  for (int k = 0; k < 3; ++k)
+    QS_HIP(hipMemcpy2DAsync(S->uold[k], row, S->old_[k], prow, row, m->Ny, hipMemcpyDeviceToDevice,
+                            stream_of(m)));
+  using StageFn = int (*)(const Qcomp*, Qcomp*, Qcomp*, const Qcomp*, const Qreal*, const Qreal*, Qreal,
+                          const Mesh*);
+  const StageFn stage[4] = {qs_rk4_func1D, qs_rk4_func2D, qs_rk4_func3D, qs_rk4_func4D};
+  for (int n = 0; n < nsteps; ++n) {
+    for (int k = 0; k < 3; ++k) QS_TRY(qs_rk4_func0D(S->uold[k], curr[k]->spec, S->unew[k], IF[k], IFh[k], dt, m));
+    for (int q = 0; q < 4; ++q) {
+      QS_TRY(unfused_rhs(S));
+      for (int k = 0; k < 3; ++k)
+        QS_TRY(stage[q](S->uold[k], curr[k]->spec, S->unew[k], nonl[k]->spec, IF[k], IFh[k], dt, m));
+    }
     for (int k = 0; k < 3; ++k) QS_TRY(qs_dealias(S->unew[k], m));
     for (int k = 0; k < 3; ++k) QS_TRY(qs_SpecSet(S->uold[k], S->unew[k], m));
   }

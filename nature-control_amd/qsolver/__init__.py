@@ -138,6 +138,7 @@ _SIGS = {
     "qs_solver_get_state": [_P, _P, _P, _P],
     "qs_step_ifrk2": [_P, _I],
     "qs_step_ifrk2_unfused": [_P, _I],
+    "qs_step_rk4_unfused": [_P, _I],
     "qs_solver_nonfinite": [_P, ctypes.POINTER(_I)],
     "qs_solver_profile": [_P, _I],
     "qs_solver_debug_phase": [_P, _I],
@@ -420,6 +421,9 @@ class Solver:
 
     def step_unfused(self, nsteps=1):
         call("qs_step_ifrk2_unfused", self._s, int(nsteps))
+
+    def step_rk4_unfused(self, nsteps=1):
+        call("qs_step_rk4_unfused", self._s, int(nsteps))
 
     KERNEL_CLASSES = ("col_prologue", "row_rhs", "col_stage1", "col_stage2")
 

@@ -151,7 +151,9 @@ struct OMesh {
   size_t np() const { return (size_t)Nx * Ny; }
 
   // Basic/Mesh.cuh:6-21 -- Nyquist row (j == Ny/2) gets the positive wavenumber.
-  void waveinit() {
+  // Host code in the reference: k^2 from two rounded products (no fused
+  // multiply-add), as an x86-64 host compiler evaluates it.
+  __attribute__((optimize("fp-contract=off"))) void waveinit() {
     for (int i = 0; i < Nxh; i++)
       for (int j = 0; j < Ny; j++) {
         int index = i + j * Nxh;
