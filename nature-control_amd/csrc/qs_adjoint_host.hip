@@ -452,6 +452,12 @@ int adj_group_backward(AdjGroup* G) {
   return group_backward(G);
 }
 int adj_group_steps(const AdjGroup* G) { return G->nsteps; }
+// Frees a run's adjoint memory without touching its Mesh (the solver's
+// destroy path: the device is synchronised first, the Mesh may be gone).
+void adj_state_free(qs_solver* S) {
+  run_free(S->adj);
+  S->adj = nullptr;
+}
 size_t adj_group_bytes(const AdjGroup* G) {
   size_t b = G->bytes;
   for (auto* S : G->runs) b += S->adj ? S->adj->bytes : 0;

@@ -86,10 +86,11 @@ int qs_mesh_create(int Nx, int Ny, Qreal Lx, Qreal Ly, int BSZ, Mesh** out) {
   std::vector<double> kx1(Nxh), ky1(Ny);
   for (int i = 0; i < Nxh; ++i) kx1[i] = kx[i];
   for (int j = 0; j < Ny; ++j) ky1[j] = ky[(size_t)j * Nxh];
-  std::vector<double2> twx(Nx), twy(Ny), tw1024(1024);
+  std::vector<double2> twx(Nx), twy(Ny), tw1024(1024), tw768(768);
   host_twiddles(Nx, twx.data());
   host_twiddles(Ny, twy.data());
   host_twiddles(1024, tw1024.data());
+  host_twiddles(768, tw768.data());
 
   auto cleanup = [&]() { qs_mesh_destroy(m); };
   {
@@ -115,13 +116,15 @@ int qs_mesh_create(int Nx, int Ny, Qreal Lx, Qreal Ly, int BSZ, Mesh** out) {
   MALLOC(m->priv->kx1, sizeof(double) * Nxh);
   MALLOC(m->priv->ky1, sizeof(double) * Ny);
   MALLOC(m->priv->tw1024, sizeof(double2) * 1024);
+  MALLOC(m->priv->tw768, sizeof(double2) * 768);
 #undef MALLOC
   struct { void* d; const void* h; size_t b; } cp[] = {
       {m->kx, kx.data(), ws}, {m->ky, ky.data(), ws}, {m->k_squared, k2.data(), ws},
       {m->cutoff, cut.data(), ws}, {m->priv->twx, twx.data(), sizeof(double2) * Nx},
       {m->priv->twy, twy.data(), sizeof(double2) * Ny}, {m->priv->kx1, kx1.data(), sizeof(double) * Nxh},
       {m->priv->ky1, ky1.data(), sizeof(double) * Ny},
-      {m->priv->tw1024, tw1024.data(), sizeof(double2) * 1024}};
+      {m->priv->tw1024, tw1024.data(), sizeof(double2) * 1024},
+      {m->priv->tw768, tw768.data(), sizeof(double2) * 768}};
   for (auto& c : cp) {
     hipError_t e = hipMemcpy(c.d, c.h, c.b, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -138,7 +141,8 @@ int qs_mesh_destroy(Mesh* m) {
   void* ptrs[] = {m->spec, m->phys, m->kx, m->ky, m->k_squared, m->cutoff,
                   m->priv ? m->priv->twx : nullptr, m->priv ? m->priv->twy : nullptr,
                   m->priv ? m->priv->kx1 : nullptr, m->priv ? m->priv->ky1 : nullptr,
-                  m->priv ? (void*)m->priv->tw1024 : nullptr, m->priv ? (void*)m->priv->diag : nullptr};
+                  m->priv ? (void*)m->priv->tw1024 : nullptr, m->priv ? (void*)m->priv->tw768 : nullptr,
+                  m->priv ? (void*)m->priv->diag : nullptr};
   if (m->priv && m->priv->diag) (void)hipStreamSynchronize(m->priv->stream);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -49,9 +49,10 @@ namespace {
 //  6. cooperative unpack + spectral combination -> F0..F6 (coalesced stores).
 template <int N>
 __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm, const double2* twq) {
-  constexpr int NV = 1024 / N;          // rows of this job (= vectors per wave FFT)
-  constexpr int LPC = N / 16;
-  constexpr int VS = N + N / 16;        // padded vector stride inside a wave buffer
+  using L = WaveLay<N>;
+  constexpr int NV = L::NV;             // rows of this job (= vectors per wave FFT)
+  constexpr int LPC = L::LPC, E = L::E;
+  constexpr int VS = L::VS;             // padded vector stride inside a wave buffer
   constexpr int H = N / 2 + 1;
   constexpr int RH = NV * H;            // staged bins per array
   const TwN<N> tw{twq};
@@ -66,7 +67,7 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
 
   // ---- 4. pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) --
   const double alu = a.alpha_p ? gload(a.alpha_p) : a.alpha;   // uniform activity of this step
-  for (int p = tid; p < 1024; p += 384) {
+  for (int p = tid; p < NV * N; p += 384) {
     const int gg = p / N, x = p % N;
     const int o = gg * VS + swz(x);
     const int yy = row0 + gg;
@@ -100,11 +101,11 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
 
   // ---- 5. forward x FFTs ------------------------------------------------------
 #pragma unroll
-  for (int m = 0; m < 16; ++m) v[m] = vb[swz(lp + LPC * m)];
+  for (int m = 0; m < E; ++m) v[m] = vb[swz(lp + LPC * m)];
   wave_sync();
   wave_fft<N, -1>(v, vb, lp, tw);
 #pragma unroll
-  for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = v[m];
+  for (int m = 0; m < E; ++m) vb[swz(lp + LPC * m)] = v[m];
   __syncthreads();
 
   // ---- 6. unpack the two-for-one spectra and form F0..F6 ---------------------
@@ -113,7 +114,8 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
     const int vec = e / H, k = e % H;
     const int y = row0 + vec;
     if (y >= a.Ny) continue;
-    const int o = vec * VS + swz(k), oc = vec * VS + swz((N - k) & (N - 1));
+    constexpr bool POW2 = (N & (N - 1)) == 0;
+    const int o = vec * VS + swz(k), oc = vec * VS + swz(POW2 ? ((N - k) & (N - 1)) : (k == 0 ? 0 : N - k));
     double2 Sp[6][2];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
@@ -144,8 +146,11 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
 // Column job: one field (FLD 0 = w, 1 = r1, 2 = r2) on a tile of 4*1024/NY
 // columns (ColTile), one column per lane group, FFTs along y per wavefront.
 template <int NY, int MODE>
-__device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq) {
-  constexpr int LPC = NY / 16;
+__device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq,
+                                        bool pro_arg) {
+  // prologue: final-stage kernels only (see is_prologue)
+  const bool pro = (MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && pro_arg;
+  constexpr int LPC = WaveLay<NY>::LPC, E = WaveLay<NY>::E;
   const TwN<NY> tw{twq};
   ColTile<NY> T(a, sm, i0);
   const int lp = T.lp;
@@ -164,8 +169,12 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   auto tile_store = [&](double2* __restrict__ dst) { T.store(dst); };
 
   double2 st[16];   // the state of the next stage (curr / new / old)
-  if (MODE == MODE_PRO) {
-    stage_in(a.old_[FLD], st);
+  if (pro) {
+    // straight from HBM in the lane layout (a prologue runs once per restart:
+    // no staging registers, which the stage path needs for its own tiles)
+    const double2* src = a.old_[FLD] + ic;
+#pragma unroll
+    for (int m = 0; m < E; ++m) st[m] = gload(src + (unsigned)((lp + LPC * m) * a.pitch));
   } else {
     // ---- N_X from the forward set ------------------------------------------
     //   N_w = F0 + i ky F1 + ky^2 F2,  N_r = F_a + i ky F_b   (in st)
@@ -177,7 +186,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       stage_in(a.mix[f0 + 1], v);
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < E; ++m) {
         const double ky = kyj(lp + LPC * m);
         st[m] = make_double2(st[m].x - ky * v[m].y, st[m].y + ky * v[m].x);
       }
@@ -187,7 +196,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       stage_in(a.mix[2], v);
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < E; ++m) {
         const double ky = kyj(lp + LPC * m);
         st[m] = make_double2(st[m].x + ky * ky * v[m].x, st[m].y + ky * ky * v[m].y);
       }
@@ -204,7 +213,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       return (i < ((a.Nxh - 1) * 2 / 3 + 1)) && ((j < NY / 3 + 1) || ((NY - j) < NY / 3));
     };
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {   // real part of N_w at the self-conjugate bins
+    for (int m = 0; m < E; ++m) {   // real part of N_w at the self-conjugate bins
       const int j = lp + LPC * m;
       if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
     }
@@ -213,7 +222,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const bool from_old = MODE == MODE_S1 || MODE == MODE_R1 || MODE == MODE_E;
       tile_load(from_old ? a.old_[FLD] : a.new_[FLD]);
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
         const double2 in = *sl;
@@ -260,7 +269,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       // R2, R3: new += 1/3 IFh dt N (new tile), then curr from old (old tile)
       tile_load(a.new_[FLD]);
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
         const double2 nw = *sl;
@@ -272,7 +281,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       tile_store(a.new_[FLD]);
       tile_load(a.old_[FLD]);
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         const double2 old = vb[swz(j)];
         const double Lk = lin(j);
@@ -295,12 +304,12 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     {
       double2 v[16];
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = st[m];
+      for (int m = 0; m < E; ++m) v[m] = st[m];
       wave_fft<NY, +1>(v, vb, lp, tw);
       stage_out(a.mout[0], v);
     }
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {   // st <- psi
+    for (int m = 0; m < E; ++m) {   // st <- psi
       const int j = lp + LPC * m;
       const double ky = kyj(j);
       const double k2 = kx * kx + ky * ky;
@@ -309,12 +318,12 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     {
       double2 v[16];
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = st[m];
+      for (int m = 0; m < E; ++m) v[m] = st[m];
       wave_fft<NY, +1>(v, vb, lp, tw);
       stage_out(a.mout[1], v);
     }
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {   // st <- i ky psi
+    for (int m = 0; m < E; ++m) {   // st <- i ky psi
       const double ky = kyj(lp + LPC * m);
       st[m] = make_double2(-ky * st[m].y, ky * st[m].x);
     }
@@ -325,12 +334,12 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     {
       double2 v[16];
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = st[m];
+      for (int m = 0; m < E; ++m) v[m] = st[m];
       wave_fft<NY, +1>(v, vb, lp, tw);
       stage_out(a.mout[m0], v);
     }
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {   // st <- -ky^2 R
+    for (int m = 0; m < E; ++m) {   // st <- -ky^2 R
       const double ky = kyj(lp + LPC * m);
       st[m] = make_double2(-ky * ky * st[m].x, -ky * ky * st[m].y);
     }
@@ -347,7 +356,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
 
 template <int N>
 __device__ __forceinline__ void row_dispatch_n(const RunDesc& a, int job, double2* sm, const double2* twq) {
-  row_job<N>(a, job * (1024 / N), sm, twq);
+  row_job<N>(a, job * WaveLay<N>::NV, sm, twq);
 }
 
 __device__ void row_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq) {
@@ -363,18 +372,18 @@ __device__ void row_dispatch(const RunDesc& a, int job, double2* sm, const doubl
 }
 
 template <int MODE>
-__device__ void col_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq) {
-  const int tc = 4 * (1024 / a.Ny);
+__device__ void col_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq, bool pro) {
+  const int tc = 4 * wave_nv(a.Ny);
   const int ntiles = (a.Nxh + tc - 1) / tc;
   const int fld = job / ntiles, i0 = (job % ntiles) * tc;
   switch (a.Ny) {
-    case 16: col_job<16, MODE>(a, fld, i0, sm, twq); break;
-    case 32: col_job<32, MODE>(a, fld, i0, sm, twq); break;
-    case 64: col_job<64, MODE>(a, fld, i0, sm, twq); break;
-    case 128: col_job<128, MODE>(a, fld, i0, sm, twq); break;
-    case 256: col_job<256, MODE>(a, fld, i0, sm, twq); break;
-    case 512: col_job<512, MODE>(a, fld, i0, sm, twq); break;
-    default: col_job<1024, MODE>(a, fld, i0, sm, twq); break;
+    case 16: col_job<16, MODE>(a, fld, i0, sm, twq, pro); break;
+    case 32: col_job<32, MODE>(a, fld, i0, sm, twq, pro); break;
+    case 64: col_job<64, MODE>(a, fld, i0, sm, twq, pro); break;
+    case 128: col_job<128, MODE>(a, fld, i0, sm, twq, pro); break;
+    case 256: col_job<256, MODE>(a, fld, i0, sm, twq, pro); break;
+    case 512: col_job<512, MODE>(a, fld, i0, sm, twq, pro); break;
+    default: col_job<1024, MODE>(a, fld, i0, sm, twq, pro); break;
   }
 }
 
@@ -397,15 +406,16 @@ __global__ __launch_bounds__(384) void k_row(const RunDesc* __restrict__ runs, c
 }
 
 template <int NY, int MODE>
-__device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const double2* twq) {
-  constexpr int tc = 4 * (1024 / NY);
+__device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const double2* twq, bool pro) {
+  constexpr int tc = 4 * WaveLay<NY>::NV;
   const int ntiles = (a.Nxh + tc - 1) / tc;
-  col_job<NY, MODE>(a, job / ntiles, (job % ntiles) * tc, sm, twq);
+  col_job<NY, MODE>(a, job / ntiles, (job % ntiles) * tc, sm, twq, pro);
 }
 
+// pro != 0: the prologue of the final-stage modes (S2, R4, E).
 template <int NFIX, int MODE>
 __global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
-                                             int nwork, const double2* __restrict__ tw1024) {
+                                             int nwork, const double2* __restrict__ tw1024, int pro) {
   __shared__ double2 sm[4 * (kBuf + 1)];
   __shared__ double2 twq[kTwq];
   stage_twiddles(twq, tw1024, threadIdx.x, 256);
@@ -413,8 +423,8 @@ __global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, c
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   for (int w = L; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
-    if constexpr (NFIX > 0) col_dispatch_n<NFIX, MODE>(runs[it.x], it.y, sm, twq);
-    else col_dispatch<MODE>(runs[it.x], it.y, sm, twq);
+    if constexpr (NFIX > 0) col_dispatch_n<NFIX, MODE>(runs[it.x], it.y, sm, twq, pro != 0);
+    else col_dispatch<MODE>(runs[it.x], it.y, sm, twq, pro != 0);
     __syncthreads();
   }
 }
@@ -430,16 +440,18 @@ template <int NFIX>
 int launch_col_n(int mode, const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw,
                  hipStream_t st) {
   switch (mode) {
-#define QS_COL_MODE(M) \
-    case M: hipLaunchKernelGGL((k_col<NFIX, M>), dim3(grid), dim3(256), 0, st, runs, work, nwork, tw); break;
-    QS_COL_MODE(MODE_PRO)
-    QS_COL_MODE(MODE_S1)
-    QS_COL_MODE(MODE_S2)
-    QS_COL_MODE(MODE_R1)
-    QS_COL_MODE(MODE_R2)
-    QS_COL_MODE(MODE_R3)
-    QS_COL_MODE(MODE_R4)
-    QS_COL_MODE(MODE_E)
+#define QS_COL_MODE(M, K, P) \
+    case M: hipLaunchKernelGGL((k_col<NFIX, K>), dim3(grid), dim3(256), 0, st, runs, work, nwork, tw, P); break;
+    QS_COL_MODE(MODE_PRO, MODE_S2, 1)
+    QS_COL_MODE(MODE_PRO_R4, MODE_R4, 1)
+    QS_COL_MODE(MODE_PRO_E, MODE_E, 1)
+    QS_COL_MODE(MODE_S1, MODE_S1, 0)
+    QS_COL_MODE(MODE_S2, MODE_S2, 0)
+    QS_COL_MODE(MODE_R1, MODE_R1, 0)
+    QS_COL_MODE(MODE_R2, MODE_R2, 0)
+    QS_COL_MODE(MODE_R3, MODE_R3, 0)
+    QS_COL_MODE(MODE_R4, MODE_R4, 0)
+    QS_COL_MODE(MODE_E, MODE_E, 0)
 #undef QS_COL_MODE
     default: return fail(QS_ERR_INVALID_ARG, "column mode %d", mode);
   }
@@ -449,14 +461,19 @@ int launch_col_n(int mode, const RunDesc* runs, const int2* work, int nwork, int
 
 }  // namespace
 
-int row_jobs(const RunDesc& a) { return (a.Ny + 1024 / a.Nx - 1) / (1024 / a.Nx); }
+int row_jobs(const RunDesc& a) { return (a.Ny + wave_nv(a.Nx) - 1) / wave_nv(a.Nx); }
 int col_jobs(const RunDesc& a) {
-  const int tc = 4 * (1024 / a.Ny);
+  const int tc = 4 * wave_nv(a.Ny);
   return 3 * ((a.Nxh + tc - 1) / tc);
 }
 
 #define QS_NFIX_SWITCH(n, FN, ...)                 \
   switch (n) {                                      \
+    case 48: return FN<48>(__VA_ARGS__);            \
+    case 96: return FN<96>(__VA_ARGS__);            \
+    case 192: return FN<192>(__VA_ARGS__);          \
+    case 384: return FN<384>(__VA_ARGS__);          \
+    case 768: return FN<768>(__VA_ARGS__);          \
     case 16: return FN<16>(__VA_ARGS__);            \
     case 32: return FN<32>(__VA_ARGS__);            \
     case 64: return FN<64>(__VA_ARGS__);            \
@@ -491,12 +508,13 @@ __global__ __launch_bounds__(64) void k_wave_fft_test(const double2* in, double2
   __shared__ double2 twq[kTwq];
   stage_twiddles(twq, tw1024, threadIdx.x, 64);
   __syncthreads();
-  constexpr int CW = 1024 / N, LPC = N / 16, VS = N + N / 16;
+  using L = WaveLay<N>;
+  constexpr int CW = L::NV, LPC = L::LPC, VS = L::VS, E = L::E;
   const int lane = threadIdx.x, lp = lane / CW, g = lane % CW;
   double2 v[16];
-  for (int m = 0; m < 16; ++m) v[m] = in[g * N + lp + LPC * m];
+  for (int m = 0; m < E; ++m) v[m] = in[g * N + lp + LPC * m];
   wave_fft<N, SIGN>(v, sm + g * VS, lp, TwN<N>{twq});
-  for (int m = 0; m < 16; ++m) out[g * N + lp + LPC * m] = v[m];
+  for (int m = 0; m < E; ++m) out[g * N + lp + LPC * m] = v[m];
 }
 template <int N>
 int launch_wave_fft_test(int sign, const double2* in, double2* out, const double2* tw, hipStream_t st) {
@@ -516,6 +534,11 @@ int wave_fft_test(int n, int sign, const double2* in, double2* out, const double
     case 256: return launch_wave_fft_test<256>(sign, in, out, tw1024, st);
     case 512: return launch_wave_fft_test<512>(sign, in, out, tw1024, st);
     case 1024: return launch_wave_fft_test<1024>(sign, in, out, tw1024, st);
+    case 48: return launch_wave_fft_test<48>(sign, in, out, tw1024, st);
+    case 96: return launch_wave_fft_test<96>(sign, in, out, tw1024, st);
+    case 192: return launch_wave_fft_test<192>(sign, in, out, tw1024, st);
+    case 384: return launch_wave_fft_test<384>(sign, in, out, tw1024, st);
+    case 768: return launch_wave_fft_test<768>(sign, in, out, tw1024, st);
     default: return fail(QS_ERR_UNSUPPORTED_SIZE, "wave fft %d", n);
   }
 }

@@ -7,7 +7,15 @@ namespace qs {
 
 // Column kernel modes: prologue (inverse set of the state), IFRK2 stages 1-2,
 // IF-RK4 stages 1-4, IF-Euler.
-enum { MODE_PRO = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_R1 = 3, MODE_R2 = 4, MODE_R3 = 5, MODE_R4 = 6, MODE_E = 7 };
+enum { MODE_PRO = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_R1 = 3, MODE_R2 = 4, MODE_R3 = 5, MODE_R4 = 6, MODE_E = 7,
+       MODE_PRO_R4 = 8, MODE_PRO_E = 9 };
+// Prologue modes (the inverse set of the state in old_, no stage update) run
+// the kernel of the integrator's final stage with its update skipped:
+// MODE_PRO that of IFRK2 stage 2, MODE_PRO_R4 that of RK4 stage 4, MODE_PRO_E
+// that of IF-Euler.  The inverse set is then computed by the same code from
+// the same values as at the end of a step, so a restart from a stored state
+// continues the run bit for bit.
+inline bool is_prologue(int mode) { return mode == MODE_PRO || mode == MODE_PRO_R4 || mode == MODE_PRO_E; }
 // Column kernels of the adjoint step (see qs_adjoint.hip).
 enum { AMODE_START = 0, AMODE_B = 1, AMODE_A = 2, AMODE_END = 3 };
 

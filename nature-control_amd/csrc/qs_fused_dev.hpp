@@ -8,14 +8,16 @@ namespace qs {
 namespace {
 
 constexpr int kBuf = 1088;           // LDS entries of one wave buffer (1024 + 1024/16)
-constexpr int kTwq = 256;            // quarter twiddle table of W_1024
+constexpr int kTwq = 256;            // staged twiddles: the quarter table of W_1024 (or of W_768)
 
-// W_N^m from the quarter table of W_1024 (N <= 1024, power of two).
+// W_N^m from the staged quarter table: of W_1024 for N = 2^a <= 1024, of W_768
+// for N = 3 * 2^a <= 768 (the kernel is handed the matching table).
 template <int N>
 struct TwN {
   const double2* p;
   __device__ __forceinline__ double2 operator()(int m) const {
-    return TwQuarter<1024>{p}(m * (1024 / N));
+    if constexpr (N % 3 == 0) return TwQuarter<768>{p}(m * (768 / N));
+    else return TwQuarter<1024>{p}(m * (1024 / N));
   }
 };
 
@@ -45,12 +47,12 @@ __device__ __forceinline__ double recip(double x) {
 }
 
 // Row staging: rows row0 .. row0+NV-1 of the 7 arrays arr[0..6] (x-spectral,
-// H = N/2+1 bins each) into sm[t*RH + vec*H + k].  RH = 512 + NV <= 576 < 2*384:
+// H = N/2+1 bins each) into sm[t*RH + vec*H + k].  RH = NV*H <= 576 < 2*384:
 // two slots per thread and array, all 14 loads in flight before the first LDS
 // write.  Ends with a barrier.
 template <int N>
 __device__ __forceinline__ void stage_rows(double2* const* arr, const RunDesc& a, int row0, double2* sm) {
-  constexpr int NV = 1024 / N, H = N / 2 + 1, RH = NV * H;
+  constexpr int NV = WaveLay<N>::NV, H = N / 2 + 1, RH = NV * H;
   static_assert(RH <= 2 * 384, "row staging assumes two slots per thread");
   const int tid = threadIdx.x;
   double2 ld[7][2];
@@ -87,7 +89,8 @@ __device__ __forceinline__ void stage_rows(double2* const* arr, const RunDesc& a
 template <int N, class TW>
 __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, double2* sm, const TW& tw,
                                                double2 (&v)[16]) {
-  constexpr int NV = 1024 / N, LPC = N / 16, VS = N + N / 16, H = N / 2 + 1, RH = NV * H;
+  using L = WaveLay<N>;
+  constexpr int NV = L::NV, LPC = L::LPC, VS = L::VS, E = L::E, H = N / 2 + 1, RH = NV * H;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;
@@ -96,7 +99,7 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
   if (wave < 5) {
     const double w00 = gload(a.w00);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
+    for (int m = 0; m < E; ++m) {
       const int k = lp + LPC * m;
       const int bin = k > N / 2 ? N - k : k;
       const double kx = a.dkx * bin;
@@ -131,19 +134,21 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
     const double sc = a.scale;
     wave_fft<N, +1>(v, vb, lp, tw);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = make_double2(v[m].x * sc, v[m].y * sc);
+    for (int m = 0; m < E; ++m) vb[swz(lp + LPC * m)] = make_double2(v[m].x * sc, v[m].y * sc);
   }
   __syncthreads();
 }
 
-// Column tile of TC = 4 * 1024/NY columns starting at i0: wave w owns columns
-// i0 + w*CW + g (g < CW = 1024/NY).  Global traffic is staged through the wave
-// buffers (stride WS) so every load / store instruction covers whole row
-// segments of the tile, while the FFTs run per wavefront on the lane layout
-// "lane (lp, g) holds rows lp + LPC*m".
+// Column tile of TC = 4 * CW columns starting at i0 (CW = WaveLay<NY>::NV):
+// wave w owns columns i0 + w*CW + g (g < CW).  Global traffic is staged
+// through the wave buffers (stride WS) so every load / store instruction
+// covers whole row segments of the tile, while the FFTs run per wavefront on
+// the lane layout "lane (lp, g) holds rows lp + LPC*m, m < E".  A tile holds
+// TC * NY = 256 * E elements: E per thread.
 template <int NY>
 struct ColTile {
-  static constexpr int CW = 1024 / NY, TC = 4 * CW, LPC = NY / 16, VS = NY + NY / 16;
+  static constexpr int CW = WaveLay<NY>::NV, TC = 4 * CW, LPC = WaveLay<NY>::LPC, VS = WaveLay<NY>::VS;
+  static constexpr int E = WaveLay<NY>::E;
   static constexpr int WS = kBuf + 1;   // odd wave stride: spreads LDS banks
   const RunDesc& a;
   double2* sm;
@@ -161,9 +166,9 @@ struct ColTile {
   // all 16 loads are in flight before the first LDS write (the scheduling
   // barrier keeps the compiler from pairing each load with its write)
   __device__ __forceinline__ void load(const double2* __restrict__ src) {
-    double2 ld[16];
+    double2 ld[E];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
+    for (int q = 0; q < E; ++q) {
       const int e = tid + 256 * q;
       const int c = e % TC, j = e / TC;
       const int ii = i0 + c;
@@ -174,7 +179,7 @@ struct ColTile {
     __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
+    for (int q = 0; q < E; ++q) {
       const int e = tid + 256 * q;
       const int c = e % TC, j = e / TC;
       sm[slot(j, c)] = i0 + c < a.Nxh ? ld[q] : make_double2(0.0, 0.0);
@@ -187,7 +192,7 @@ struct ColTile {
   __device__ __forceinline__ void store(double2* __restrict__ dst) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
+    for (int q = 0; q < E; ++q) {
       const int e = tid + 256 * q;
       const int c = e % TC, j = e / TC;
       const int ii = i0 + c;
@@ -198,12 +203,12 @@ struct ColTile {
   __device__ __forceinline__ void stage_in(const double2* __restrict__ src, double2 (&v)[16]) {
     load(src);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = vb[swz(lp + LPC * m)];
+    for (int m = 0; m < E; ++m) v[m] = vb[swz(lp + LPC * m)];
     wave_sync();
   }
   __device__ __forceinline__ void stage_out(double2* __restrict__ dst, const double2 (&v)[16]) {
 #pragma unroll
-    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = v[m];
+    for (int m = 0; m < E; ++m) vb[swz(lp + LPC * m)] = v[m];
     store(dst);
   }
 };

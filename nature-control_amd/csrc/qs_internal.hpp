@@ -39,6 +39,7 @@ inline bool supported_side(int n) {
   return (is_pow2(n) && n >= 16 && n <= 2048) || (n % 3 == 0 && is_pow2(n / 3) && n >= 48 && n <= 1536);
 }
 inline bool fused_side(int n) { return is_pow2(n) && n >= 16 && n <= 1024; }
+inline bool fused_tri_side(int n) { return n % 3 == 0 && is_pow2(n / 3) && n >= 48 && n <= 768; }
 inline int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
 
 }  // namespace qs
@@ -51,6 +52,7 @@ struct qs_mesh_priv {
   double* kx1 = nullptr;    // kx(i), i < Nxh   (1-D, separable tables)
   double* ky1 = nullptr;    // ky(j), j < Ny
   double2* tw1024 = nullptr; // W_1024^m, m < 1024 (fused kernels use the first 256)
+  double2* tw768 = nullptr;  // W_768^m, m < 768 (fused kernels of 3 * 2^a sides use the first 192)
   int ncu = 0;               // compute units of the device the mesh was created on
   double* diag = nullptr;    // partial sums of the diagnostics (qs_diag.hip), grown on demand
   size_t diag_n = 0;
@@ -348,9 +350,66 @@ __device__ __forceinline__ void dft16(double2* v) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Radix-12 DFT in registers: 3 x 4 with the W12 twiddles between the stages
+// (n = n2 + 4 n1, k = k1 + 3 k2).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 w12(int m, int sign) {
+  constexpr double C = 0.86602540378443864676;   // cos(pi/6)
+  double c = 1.0, s = 0.0;                        // W12^m = c - i s (forward)
+  switch (m) {
+    case 1: c = C; s = 0.5; break;
+    case 2: c = 0.5; s = C; break;
+    case 3: c = 0.0; s = 1.0; break;
+    case 4: c = -0.5; s = C; break;
+    case 6: c = -1.0; s = 0.0; break;
+    default: break;
+  }
+  return make_double2(c, sign < 0 ? -s : s);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft12(double2* v) {
+  double2 a[4][3];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    double2 t[3] = {v[n2], v[n2 + 4], v[n2 + 8]};
+    dft3<SIGN>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) a[n2][k1] = (n2 * k1 == 0) ? t[k1] : cmul(t[k1], w12(n2 * k1, SIGN));
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    double2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+    dft4<SIGN>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 3 * k2] = t[k2];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lane layout of the per-wavefront FFT of length-N vectors: E elements per
+// lane, LPC = N/E lanes per vector, NV = 64/LPC vectors per wave (one wave
+// holds NV whole vectors), vector stride VS in a wave buffer (swizzled).
+//   N = 2^a (16..1024):      E = 16, radix-16 first pass, radix-16 passes;
+//   N = 3 * 2^a (48..768):   E = 12, radix-12 first pass, radix-4/2 passes.
+// ---------------------------------------------------------------------------
+template <int N>
+struct WaveLay {
+  static constexpr bool TRI = N % 3 == 0;
+  static constexpr int E = TRI ? 12 : 16;
+  static constexpr int LPC = N / E;
+  static constexpr int NV = 64 / LPC;
+  static constexpr int VS = N + N / 16;
+  static constexpr int RMAX = TRI ? 4 : 16;   // radix of the LDS passes
+  static_assert(LPC * NV == 64 && LPC * E == N, "one wave holds NV whole vectors");
+};
+constexpr int wave_nv(int N) { return N % 3 == 0 ? 768 / N : 1024 / N; }
+
 template <int R, int SIGN>
 __device__ __forceinline__ void dft_any(double2* v) {
   if constexpr (R == 16) dft16<SIGN>(v);
+  else if constexpr (R == 12) dft12<SIGN>(v);
   else dft_small<R, SIGN>(v);
 }
 
@@ -371,27 +430,28 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ int swz(int e) { return e + (e >> 4); }
 
 // ---------------------------------------------------------------------------
-// Per-wavefront FFT of CW = 1024/N independent length-N vectors (16 <= N <=
-// 1024).  Lane (lp, g) with lp < LPC = N/16 holds elements lp + LPC*m,
-// m < 16, of vector g in v[]; on return it holds the transform at the same
-// positions.  Radix-16 first pass in registers, then radix-min(16, N/P)
-// Stockham passes through `buf` (this wave's LDS region, CW*(N + N/16)
-// entries, vector g at offset g*(N + N/16)).  Only wave-level ordering is
-// needed: no workgroup barrier.
+// Per-wavefront FFT of NV independent length-N vectors (WaveLay<N>).  Lane
+// (lp, g) with lp < LPC holds elements lp + LPC*m, m < E, of vector g in v[];
+// on return it holds the transform at the same positions.  Radix-E first pass
+// in registers, then Stockham passes of radix min(RMAX, N/P) through `buf`
+// (this lane's vector in its wave's LDS buffer, VS entries).  Only wave-level
+// ordering is needed: no workgroup barrier.
 // ---------------------------------------------------------------------------
 template <int N, int P, int SIGN, class TW>
 __device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, int lp, const TW& tw) {
-  constexpr int LPC = N / 16;
-  constexpr int R = (N / P >= 16) ? 16 : N / P;
+  using L = WaveLay<N>;
+  constexpr int LPC = L::LPC;
+  constexpr int R = (N / P >= L::RMAX) ? L::RMAX : N / P;
   constexpr int Q = N / R;
   constexpr int BPL = Q / LPC;          // butterflies per lane
+  static_assert(BPL * R == L::E && N % (P * R) == 0, "pass shape");
   constexpr bool LAST = (P * R == N);
   constexpr int STEP = N / (P * R);
   double2 t[BPL][R];
 #pragma unroll
   for (int q = 0; q < BPL; ++q) {
     const int b = lp + LPC * q;
-    const int k = b & (P - 1);
+    const int k = (int)((unsigned)b % (unsigned)P);   // b >= 0: an AND for P = 2^a
 #pragma unroll
     for (int r = 0; r < R; ++r) t[q][r] = buf[swz(b + r * Q)];
     // Twiddles w^r, w = W^(k STEP): the powers w^1, w^2, w^4, w^8 come from the
@@ -427,7 +487,7 @@ __device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, in
 #pragma unroll
     for (int q = 0; q < BPL; ++q) {
       const int b = lp + LPC * q;
-      const int k = b & (P - 1);
+      const int k = (int)((unsigned)b % (unsigned)P);   // b >= 0: an AND for P = 2^a
 #pragma unroll
       for (int r = 0; r < R; ++r) buf[swz((b - k) * R + k + r * P)] = t[q][r];
     }
@@ -438,12 +498,13 @@ __device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, in
 
 template <int N, int SIGN, class TW>
 __device__ __forceinline__ void wave_fft(double2 (&v)[16], double2* buf, int lp, const TW& tw) {
-  dft16<SIGN>(v);                          // pass 1 (P = 1): butterfly lp
-  if constexpr (N > 16) {
+  constexpr int E = WaveLay<N>::E;
+  dft_any<E, SIGN>(v);                     // pass 1 (P = 1): butterfly lp
+  if constexpr (N > E) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) buf[swz(lp * 16 + r)] = v[r];
+    for (int r = 0; r < E; ++r) buf[swz(lp * E + r)] = v[r];
     wave_sync();
-    wave_fft_pass<N, 16, SIGN>(v, buf, lp, tw);
+    wave_fft_pass<N, E, SIGN>(v, buf, lp, tw);
     wave_sync();                           // buf may be rewritten by the caller
   }
 }

@@ -59,24 +59,29 @@ int row_grid(const qs_solver* S) {
   const int ncu = S->mesh->priv->ncu;
   return ncu > 0 ? std::min(S->nrow, ncu) : S->nrow;
 }
-// The fused kernels are instantiated for power-of-two sides up to 1024; other
-// meshes (3 * 2^a, 2048) run the per-op path (qs_step_ifrk2_unfused).
+// The fused kernels are instantiated for power-of-two sides up to 1024 and
+// for square meshes of side 3 * 2^a up to 768; other meshes run the per-op
+// path (qs_step_ifrk2_unfused, qs_step_rk4_unfused).
 int fused_check(const qs_solver* S) {
   const Mesh* m = S->mesh;
-  if (!fused_side(m->Nx) || !fused_side(m->Ny))
-    return fail(QS_ERR_UNSUPPORTED_SIZE, "fused step: %dx%d mesh (powers of two up to 1024; qs_step_ifrk2_unfused "
-                "runs any supported mesh)", m->Nx, m->Ny);
+  const bool pow2 = fused_side(m->Nx) && fused_side(m->Ny);
+  const bool tri = m->Nx == m->Ny && fused_tri_side(m->Nx);
+  if (!pow2 && !tri)
+    return fail(QS_ERR_UNSUPPORTED_SIZE, "fused step: %dx%d mesh (powers of two up to 1024, or square 3*2^a up to "
+                "768; the unfused steps run any supported mesh)", m->Nx, m->Ny);
   return QS_OK;
 }
+// The staged twiddle table of the fused kernels: W_768 for 3 * 2^a sides.
+const double2* fused_tw(const Mesh* m) { return m->Nx % 3 == 0 ? m->priv->tw768 : m->priv->tw1024; }
 int row_rhs_raw(qs_solver* S) {
   QS_TRY(fused_check(S));
-  return launch_fused_row(nfix_of(S->mesh), S->d_desc, S->d_work, S->nrow, row_grid(S), S->mesh->priv->tw1024,
+  return launch_fused_row(nfix_of(S->mesh), S->d_desc, S->d_work, S->nrow, row_grid(S), fused_tw(S->mesh),
                           stream_of(S->mesh));
 }
 int col_stage_raw(qs_solver* S, int mode) {
   QS_TRY(fused_check(S));
-  return launch_fused_col(nfix_of(S->mesh), mode, S->d_desc, S->d_work, S->ncol, S->ncol,
-                          S->mesh->priv->tw1024, stream_of(S->mesh));
+  return launch_fused_col(nfix_of(S->mesh), mode, S->d_desc, S->d_work, S->ncol, S->ncol, fused_tw(S->mesh),
+                          stream_of(S->mesh));
 }
 
 hipEvent_t take_event(qs_solver* S) {
@@ -108,7 +113,7 @@ int timed(qs_solver* S, int cls, F launch) {
 int row_rhs(qs_solver* S) { return timed(S, K_ROW, [&] { return row_rhs_raw(S); }); }
 int col_stage(qs_solver* S, int mode) {
   // intermediate stages (IFRK2 1, RK4 1-3) and final stages (IFRK2 2, RK4 4, IFE)
-  const int cls = mode == MODE_PRO ? K_COL_PRO
+  const int cls = is_prologue(mode) ? K_COL_PRO
                   : (mode == MODE_S1 || mode == MODE_R1 || mode == MODE_R2 || mode == MODE_R3) ? K_COL_S1
                                                                                             : K_COL_S2;
   return timed(S, cls, [&] { return col_stage_raw(S, mode); });
@@ -221,7 +226,7 @@ int qs_solver_create(Mesh* mesh, const qs_params* params, qs_solver** out) {
 int qs_solver_destroy(qs_solver* S) {
   if (!S) return QS_OK;
   (void)hipDeviceSynchronize();   // the Mesh may already be gone: do not touch it
-  qs_adj_release(S);
+  adj_state_free(S);
   for (int k = 0; k < 3; ++k) {
     if (S->old_[k]) (void)hipFree(S->old_[k]);
     if (S->new_[k]) (void)hipFree(S->new_[k]);
@@ -310,7 +315,7 @@ int qs_step_rk4(qs_solver* S, int nsteps) {
   if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
   for (int n = 0; n < nsteps; ++n) {
     if (!S->mixed_valid) {
-      QS_TRY(col_stage(S, MODE_PRO));
+      QS_TRY(col_stage(S, MODE_PRO_R4));
       S->mixed_valid = true;
     }
     for (int mode : {MODE_R1, MODE_R2, MODE_R3, MODE_R4}) {
@@ -328,7 +333,7 @@ int qs_step_ife(qs_solver* S, int nsteps) {
   if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
   for (int n = 0; n < nsteps; ++n) {
     if (!S->mixed_valid) {
-      QS_TRY(col_stage(S, MODE_PRO));
+      QS_TRY(col_stage(S, MODE_PRO_E));
       S->mixed_valid = true;
     }
     QS_TRY(row_rhs(S));
@@ -440,7 +445,8 @@ int qs_solver_debug_array(qs_solver* S, int which, Qcomp* out) {
 int qs_debug_wave_fft(const Mesh* mesh, int n, int sign, const Qcomp* in, Qcomp* out) {
   QS_TRY(check_mesh(mesh));
   if (!in || !out) return fail(QS_ERR_INVALID_ARG, "null argument");
-  return wave_fft_test(n, sign, (const double2*)in, (double2*)out, mesh->priv->tw1024, stream_of(mesh));
+  return wave_fft_test(n, sign, (const double2*)in, (double2*)out,
+                       n % 3 == 0 ? mesh->priv->tw768 : mesh->priv->tw1024, stream_of(mesh));
 }
 
 int qs_solver_profile(qs_solver* S, int enable) {

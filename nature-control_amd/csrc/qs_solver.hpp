@@ -76,6 +76,7 @@ void adj_group_free(AdjGroup* G);
 int adj_group_forward(AdjGroup* G, int nsteps, int k, const double* alpha_t);
 int adj_group_backward(AdjGroup* G);
 int adj_group_steps(const AdjGroup* G);
+void adj_state_free(qs_solver* S);
 size_t adj_group_bytes(const AdjGroup* G);
 int adj_run_seed(qs_solver* S, int kind, double* loss);
 }  // namespace qs

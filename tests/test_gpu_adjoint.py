@@ -131,42 +131,54 @@ def test_activity_schedule_gradient_vs_cpu_adjoint():
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
-    """Recomputed segments reproduce the forward bit for bit."""
+    """Recomputed segments reproduce the forward: the same loss and gradient
+    for checkpoint intervals 1, 2, 4 and none.  A segment restarts from its
+    checkpoint through the column prologue, a different kernel instantiation
+    from the stage that chains the inverse set in the forward, so the two agree
+    to rounding (1e-12 relative), not necessarily bit for bit."""
     print("[gpu-test] adj checkpoint invariance", flush=True)
     N, dt, n = 64, 1e-3, 9
     _, X, Ra, _ = case(N, dt, seed=11)
     ref = gpu_gradient(N, dt, X, Ra, n, 0, Q.ENSTROPHY)
     for k in (1, 2, 4):
         got = gpu_gradient(N, dt, X, Ra, n, k, Q.ENSTROPHY)
-        assert got[0] == ref[0]
-        assert np.array_equal(got[1], ref[1]), k
+        assert got[0] == pytest.approx(ref[0], rel=1e-13)
+        assert rel(got[1], ref[1]) < 1e-12, k
 
 
-@pytest.mark.parametrize("overlap", ["0", "1"])
-def test_overlapped_recompute_and_repeated_backward(monkeypatch, overlap):
+@pytest.mark.parametrize("k", [1, 3, 4])
+def test_overlapped_recompute_and_repeated_backward(monkeypatch, k):
     """Segments recomputed on a second stream beside the reverse sweep
-    (double-buffered stage slots, QS_ADJ_OVERLAP=1) or in turn with it on one
-    stream (QS_ADJ_OVERLAP=0) give the one-segment gradient bit for bit, and a
-    second reverse sweep after the same forward (whose last segment's slots the
-    first sweep may have overwritten) gives it again."""
-    print(f"[gpu-test] adj overlap={overlap}", flush=True)
+    (double-buffered stage slots, QS_ADJ_OVERLAP=1) give the gradient of the
+    same segments recomputed in turn with the sweep on one stream
+    (QS_ADJ_OVERLAP=0) bit for bit: the same kernels on the same data.  A
+    second reverse sweep after the same forward (after the first overwrote the
+    last segment's recorded stages, when there are more segments than
+    buffers) repeats it bit for bit too."""
+    print(f"[gpu-test] adj overlap k={k}", flush=True)
     N, dt, n = 64, 1e-3, 11
     _, X, Ra, _ = case(N, dt, seed=13)
     monkeypatch.delenv("QS_ADJ_OVERLAP", raising=False)
-    ref = gpu_gradient(N, dt, X, Ra, n, 0, Q.ENSTROPHY)
-    monkeypatch.setenv("QS_ADJ_OVERLAP", overlap)
-    for k in (1, 3, 4):
+    one = gpu_gradient(N, dt, X, Ra, n, 0, Q.ENSTROPHY)
+    out = {}
+    for overlap in ("0", "1"):
+        monkeypatch.setenv("QS_ADJ_OVERLAP", overlap)
         mesh, s = gpu_solver(N, dt)
         s.set_state(*(cu(x) for x in X))
         s.set_activity(cu(Ra))
         s.adj_forward(n, k)
         L, Wb = Q.adj_loss(mesh, s.get_state()[0], Q.ENSTROPHY)
-        assert L == ref[0]
         z = torch.zeros_like(Wb)
-        for rep in range(2):
+        reps = []
+        for _ in range(2):
             ab, _, _ = s.adj_backward(Wb, z, z.clone(), want_field=True)
-            assert np.array_equal(ab.cpu().numpy(), ref[1]), (k, rep)
+            reps.append(ab.cpu().numpy())
         s.close()
+        assert np.array_equal(reps[0], reps[1]), overlap
+        assert L == pytest.approx(one[0], rel=1e-13)
+        assert rel(reps[0], one[1]) < 1e-12
+        out[overlap] = reps[0]
+    assert np.array_equal(out["0"], out["1"])
 
 
 def test_gradient_vs_finite_differences_on_gpu():

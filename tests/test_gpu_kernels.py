@@ -19,13 +19,15 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("N", [16, 32, 64, 128, 256, 512, 1024])
+@pytest.mark.parametrize("N", [16, 32, 64, 128, 256, 512, 1024, 48, 96, 192, 384, 768])
 def test_wave_fft(N):
-    """Per-wavefront FFT: 1024/N vectors, both directions, all twiddle quadrants."""
+    """Per-wavefront FFT: one wave's vectors (1024/N of them for N = 2^a,
+    768/N for N = 3 * 2^a), both directions, all twiddle quadrants."""
     print(f"[gpu-test] wave fft N={N}", flush=True)
     m = Q.Mesh(64, 64, 2 * np.pi)
     rng = np.random.default_rng(N)
-    x = rng.standard_normal((1024 // N, N)) + 1j * rng.standard_normal((1024 // N, N))
+    nv = 768 // N if N % 3 == 0 else 1024 // N
+    x = rng.standard_normal((nv, N)) + 1j * rng.standard_normal((nv, N))
     for sign in (-1, 1):
         d = torch.from_numpy(x.copy()).cuda()
         o = torch.zeros_like(d)
@@ -34,7 +36,7 @@ def test_wave_fft(N):
         assert np.abs(o.cpu().numpy() - ref).max() < 1e-14 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("N", [32, 128, 256, 1024])
+@pytest.mark.parametrize("N", [32, 128, 256, 1024, 96, 768])
 def test_fused_phases(N):
     """Column prologue -> inverse set; row pass -> forward set; column stage 1
     -> IFRK2 partial sums, against the numpy model of each kernel."""

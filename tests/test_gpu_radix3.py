@@ -1,7 +1,8 @@
 """Meshes with sides 3 * 2^a (the reference driver's production grid is
 768 = 512 * 3/2, friction.cu:109): per-op transforms with a radix-3 Stockham
 pass, the wave tables against the reference's own 768^2 fixture, and the
-unfused IFRK2 and RK4 steps against the oracle (mixed-radix FFT)."""
+unfused and fused IFRK2 / RK4 / IFE steps against the oracle (mixed-radix
+FFT)."""
 import json
 import os
 
@@ -105,10 +106,35 @@ def test_unfused_steps_3x2a_vs_oracle(method, Nx, Ny, nsteps):
     (g.step_rk4_unfused if method == "rk4" else g.step_unfused)(nsteps)
     for a, b in zip((host(t) for t in g.get_state()), ref):
         assert rel_l2(a, b) < 1e-12
-    # the fused kernels are not instantiated for these sides: an error, no launch
-    with pytest.raises(Q.QsolverError) as e:
-        g.step(1)
-    assert e.value.status == Q.QS_ERR_UNSUPPORTED_SIZE
+    if Nx != Ny:   # fused kernels of 3 * 2^a sides are square only: an error, no launch
+        with pytest.raises(Q.QsolverError) as e:
+            g.step(1)
+        assert e.value.status == Q.QS_ERR_UNSUPPORTED_SIZE
+    g.close()
+
+
+@pytest.mark.parametrize("method,N,nsteps", [("ifrk2", 48, 10), ("ifrk2", 96, 20), ("ifrk2", 192, 5),
+                                             ("ifrk2", 384, 3), ("ifrk2", 768, 2), ("rk4", 96, 6),
+                                             ("rk4", 768, 1), ("ife", 96, 10)])
+def test_fused_steps_3x2a_vs_oracle(method, N, nsteps):
+    """The fused kernels on square 3 * 2^a meshes (radix-12 first pass, then
+    radix-4/2 passes per wavefront) against the oracle."""
+    print(f"[gpu-test] fused {method} {N}^2", flush=True)
+    om = O.Mesh(N, N, 2 * np.pi)
+    q11, q12 = init_q(N, seed=5)
+    rng = np.random.default_rng(6)
+    st = [om.fwd(0.1 * rng.standard_normal((N, N))), om.fwd(q11), om.fwd(q12)]
+    s = O.Solver(om, oparams(CONFIG, 1e-3))
+    s.set_state(*st)
+    s.step(nsteps, {"ifrk2": O.IFRK2, "rk4": O.RK4, "ife": O.IFE}[method])
+    ref = s.get_state()
+    m = Q.Mesh(N, N, 2 * np.pi)
+    g = Q.Solver(m, qparams(CONFIG, 1e-3))
+    g.set_state(*(dev(a) for a in st))
+    {"ifrk2": g.step, "rk4": g.step_rk4, "ife": g.step_ife}[method](nsteps)
+    assert not g.nonfinite()
+    for a, b in zip((host(t) for t in g.get_state()), ref):
+        assert rel_l2(a, b) < 1e-12
     g.close()
 
 
@@ -129,8 +155,15 @@ def test_reference_driver_grid_rk4_768():
     g.set_state(*(dev(a) for a in st))
     g.step_rk4_unfused(2)
     got = [host(t) for t in g.get_state()]
-    for a, b in zip(got[1:], ref[1:]):
-        assert rel_l2(a, b) < 1e-12
-    # omega starts at 0 and is driven by the Q stress: compare on the Q scale
-    assert np.linalg.norm(got[0] - ref[0]) <= 1e-12 * np.linalg.norm(ref[1])
+    # the same two steps fused
+    f = Q.Solver(m, qparams(DRIVER, dt))
+    f.set_state(*(dev(a) for a in st))
+    f.step_rk4(2)
+    fused = [host(t) for t in f.get_state()]
+    for x in (got, fused):
+        for a, b in zip(x[1:], ref[1:]):
+            assert rel_l2(a, b) < 1e-12
+        # omega starts at 0 and is driven by the Q stress: compare on the Q scale
+        assert np.linalg.norm(x[0] - ref[0]) <= 1e-12 * np.linalg.norm(ref[1])
     g.close()
+    f.close()
