@@ -23,8 +23,9 @@
  *    need f != ft and preserve their input, except BwdTrans's documented
  *    column-0 symmetrisation of ft (FldOp/QFldfuncs.cuh:50-55).
  *  - Supported meshes: Nx, Ny = 2^a in [16, 2048] or 3 * 2^a in [48, 1536]
- *    (the per-op calls and the unfused steps); the fused solver, its adjoint
- *    and ensembles take powers of two up to 1024.
+ *    (the per-op calls and the unfused steps); the fused steps take powers of
+ *    two up to 1024 and square 3 * 2^a meshes up to 768 (the reference
+ *    driver's grid); the adjoint and ensembles take powers of two up to 1024.
  */
 #ifndef QSOLVER_H
 #define QSOLVER_H
@@ -203,8 +204,7 @@ int qs_step_rk4(qs_solver* s, int nsteps);
 int qs_step_ife(qs_solver* s, int nsteps);
 /* Same step built from the per-op entry points above (reference call
  * granularity); for parity and for timing the unfused structure.  Runs on
- * every supported mesh, including 3 * 2^a sides such as the reference
- * driver's 768 (the fused steps take powers of two up to 1024). */
+ * every supported mesh. */
 int qs_step_ifrk2_unfused(qs_solver* s, int nsteps);
 /* The reference driver's RK4 step loop itself (friction.cu:212-260 with
  * TimeIntegration/RK4.cuh:12-74) through the per-op entry points. */
@@ -227,8 +227,9 @@ int qs_solver_profile_read(qs_solver* s, double* avg_ms, int* counts);
  * 7..9 state w/r1/r2, 10..12 stage-1 partial sums). */
 int qs_solver_debug_phase(qs_solver* s, int phase);
 int qs_solver_debug_array(qs_solver* s, int which, Qcomp* out);
-/* Test hook: the fused kernels' per-wavefront FFT (length n, 16 <= n <= 1024,
- * sign -1 forward / +1 inverse, unnormalised) on 1024/n contiguous vectors. */
+/* Test hook: the fused kernels' per-wavefront FFT (length n = 2^a in
+ * [16, 1024] or 3 * 2^a in [48, 768], sign -1 forward / +1 inverse,
+ * unnormalised) on the 1024/n (resp. 768/n) contiguous vectors of one wave. */
 int qs_debug_wave_fft(const Mesh* mesh, int n, int sign, const Qcomp* in, Qcomp* out);
 
 /* ---- discrete adjoint of the IFRK2 step ----------------------------------
