@@ -50,6 +50,11 @@ def parse():
                     help="only run the batched adjoint workload and print its line")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the discrete adjoint workload and print its line")
+    ap.add_argument("--driver-grid", type=int, default=1,
+                    help="also time the reference driver's own configuration (768^2, L=66pi, RK4, dt=0.002), "
+                         "fused and through the per-op calls")
+    ap.add_argument("--driver-grid-only", action="store_true",
+                    help="only run the reference-driver configuration and print its line")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
                          "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -298,8 +303,52 @@ def cpu_baseline(N, dt, nsteps, q11, q12):
                       f"{el:.2f} s wall on {O.get_threads()} OpenMP threads"}
 
 
+def driver_grid_bench(fused_steps=100, unfused_steps=10):
+    """The reference driver's production configuration (InertialFriction/
+    friction/friction.cu:100-112, 212-260): 768^2, L = 66 pi, lambda = 0.1,
+    Re = Er = 0.1, Rf = 7.5e-4, alpha = 0.2, dt = 0.002, IF-RK4.  Uniform
+    +-0.01 Q (splitmix64 seed 1), omega = 0.  steps/s of the fused RK4 step and
+    of the same step through the per-op entry points (the reference's call
+    granularity), and the same two for IFRK2."""
+    import numpy as np
+    import torch
+    import qsolver as Q
+    N, L, dt = 768, 33 * 2 * np.pi, 0.002
+    drv = dict(lam=0.1, Re=0.1, Er=0.1, Rf=0.00075, alpha=0.2)
+    mesh = Q.Mesh(N, N, L)
+    u = Q.synth_uniform(1, -0.01, 0.01, 2 * N * N)
+    st = [mesh.spec(), mesh.spec(), mesh.spec()]
+    mesh.FwdTrans(st[1], torch.from_numpy(u[:N * N].reshape(N, N)).cuda())
+    mesh.FwdTrans(st[2], torch.from_numpy(u[N * N:].reshape(N, N)).cuda())
+    s = Q.Solver(mesh, Q.params(dt=dt, **drv))
+
+    def rate(fn, n):
+        s.set_state(*st)
+        fn(2)   # warm-up (first-call set-up of the per-op fields and tables)
+        s.set_state(*st)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(n)
+        torch.cuda.synchronize()
+        return n / (time.perf_counter() - t0)
+    out = {"workload": "reference driver configuration: 768x768 FP64, L=66pi, lambda=0.1, Re=Er=0.1, Rf=7.5e-4, "
+                       "alpha=0.2, dt=0.002, uniform Q +-0.01 (seed 1), omega=0",
+           "rk4_fused_steps_per_s": round(rate(s.step_rk4, fused_steps), 1),
+           "rk4_unfused_steps_per_s": round(rate(s.step_rk4_unfused, unfused_steps), 2),
+           "ifrk2_fused_steps_per_s": round(rate(s.step, fused_steps), 1),
+           "ifrk2_unfused_steps_per_s": round(rate(s.step_unfused, unfused_steps), 2)}
+    out["rk4_fused_over_unfused"] = round(out["rk4_fused_steps_per_s"] / out["rk4_unfused_steps_per_s"], 1)
+    out["finite"] = not s.nonfinite()
+    out["timing"] = f"{fused_steps} fused / {unfused_steps} unfused steps, wall clock between syncs"
+    s.close()
+    return out
+
+
 def main():
     args = parse()
+    if args.driver_grid_only:
+        print(json.dumps({"driver_grid": driver_grid_bench()}), flush=True)
+        return
     if args.copy_calibrate:
         copy_calibrate(args.copy_calibrate)
         return
@@ -387,6 +436,7 @@ def main():
     adj = adjoint_bench() if args.adjoint else None
     ens = ensemble_bench() if args.ensemble else None
     badj = batched_adjoint_bench() if args.batched_adjoint else None
+    drv = driver_grid_bench() if args.driver_grid else None
     cpu = None
     if args.cpu_steps > 0:
         cpu = cpu_baseline(N, dt, args.cpu_steps, q11, q12)
@@ -417,6 +467,7 @@ def main():
         "adjoint": adj,
         "ensemble": ens,
         "batched_adjoint": badj,
+        "driver_grid": drv,
     }
     print(json.dumps(out), flush=True)
 
