@@ -11,7 +11,7 @@
 //        of the caller's spectrum) -> mixed; row pass (C2R, x 1/(Nx*Ny)) -> f
 // The C2R row pass discards the imaginary parts of the DC and Nyquist bins,
 // which fixes the semantics for non-Hermitian input (see qsolver.h).
-#include "qs_internal.hpp"
+#include "qs_fused_dev.hpp"
 
 namespace qs {
 namespace {
@@ -135,6 +135,70 @@ __global__ __launch_bounds__(ColCfg<NY>::NT) void k_col_fft(const double2* in, d
   }
 }
 
+// y-direction FFT with the per-wavefront FFT of the fused kernels
+// (WaveLay<NY>): a block of WPB wavefronts transforms TC = WPB * CW adjacent
+// columns (CW = the vectors one wave holds), loaded and stored as whole row
+// segments through LDS.  About Nxh / TC blocks, so the launch fills the device
+// (the block-wide Stockham kernel above gives one block per 4096/NY columns),
+// and the passes need only wave-level ordering.  SYM as in k_col_fft.
+template <int NY>
+struct WColCfg {
+  static constexpr int WPB = NY >= 768 ? 2 : 1;            // wavefronts per block
+  static constexpr int CW = WaveLay<NY>::NV, TC = WPB * CW, NT = 64 * WPB;
+  static constexpr int WS = kBuf + 1;                       // odd wave stride
+};
+
+template <int NY, int SIGN, bool SYM>
+__global__ __launch_bounds__(WColCfg<NY>::NT) void k_col_wfft(const double2* in, double2* out, int Nxh,
+                                                              const double2* __restrict__ twsrc, double2* sym) {
+  using C = WColCfg<NY>;
+  using L = WaveLay<NY>;
+  constexpr int CW = C::CW, TC = C::TC, NT = C::NT, WS = C::WS, LPC = L::LPC, VS = L::VS, E = L::E;
+  __shared__ double2 sm[C::WPB * WS];
+  __shared__ double2 twq[kTwq];
+  const int tid = threadIdx.x;
+  stage_twiddles(twq, twsrc, tid, NT);
+  const int i0 = blockIdx.x * TC;
+  auto slot = [&](int j, int c) { return (c / CW) * WS + (c % CW) * VS + swz(j); };
+  // TC * NY = 64 * WPB * E elements: E per thread, all loads in flight first
+  double2 ld[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tid + NT * q, c = e % TC, j = e / TC, i = i0 + c;
+    ld[q] = i < Nxh ? in[(size_t)j * Nxh + i] : make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tid + NT * q;
+    sm[slot(e / TC, e % TC)] = ld[q];
+  }
+  __syncthreads();
+  if (SYM && i0 == 0) {   // block-uniform: column 0 is tile column 0
+    for (int j = 1 + tid; j < NY / 2; j += NT) {
+      const double2 a = sm[slot(j, 0)], b = cconj(sm[slot(NY - j, 0)]);
+      const double2 mm = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+      sym[(size_t)j * Nxh] = mm;
+      sym[(size_t)(NY - j) * Nxh] = cconj(mm);
+    }
+    __syncthreads();   // the FFT below rewrites column 0's buffer
+  }
+  const int wave = tid >> 6, lane = tid & 63, lp = lane / CW, g = lane % CW;
+  double2* vb = sm + wave * WS + g * VS;
+  double2 v[16];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = vb[swz(lp + LPC * m)];
+  wave_sync();
+  wave_fft<NY, SIGN>(v, vb, lp, TwN<NY>{twq});
+#pragma unroll
+  for (int m = 0; m < E; ++m) vb[swz(lp + LPC * m)] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tid + NT * q, c = e % TC, j = e / TC, i = i0 + c;
+    if (i < Nxh) out[(size_t)j * Nxh + i] = sm[slot(j, c)];
+  }
+}
+
 template <int N>
 int launch_row_r2c(const Mesh* m, const double* f, double2* mix) {
   using C = RowCfg<N>;
@@ -158,6 +222,15 @@ int launch_row_c2r(const Mesh* m, const double2* mix, double* f) {
 
 template <int NY, int SIGN, bool SYM>
 int launch_col(const Mesh* m, const double2* in, double2* out, double2* sym) {
+  if constexpr ((NY % 3 == 0) ? NY <= 768 : NY <= 1024) {   // sizes the per-wave FFT takes
+    using W = WColCfg<NY>;
+    const int grid = (m->Nxh + W::TC - 1) / W::TC;
+    const double2* tw = NY % 3 == 0 ? m->priv->tw768 : m->priv->tw1024;
+    hipLaunchKernelGGL((k_col_wfft<NY, SIGN, SYM>), dim3(grid), dim3(W::NT), 0, stream_of(m), in, out, m->Nxh,
+                       tw, sym);
+    QS_CHECK_LAUNCH();
+    return QS_OK;
+  }
   using C = ColCfg<NY>;
   const int grid = (m->Nxh + C::TC - 1) / C::TC;
   hipLaunchKernelGGL((k_col_fft<NY, SIGN, SYM>), dim3(grid), dim3(C::NT), 0, stream_of(m), in, out,
