@@ -371,11 +371,36 @@ __device__ void row_dispatch(const RunDesc& a, int job, double2* sm, const doubl
   }
 }
 
+// Column job index -> (field, tile).  The column kernels remap block ids so
+// that each XCD takes a contiguous range of about 3 * ntiles / 8 jobs; jobs
+// are therefore numbered in groups of G = ntiles / 8 tiles, each group listing
+// the vorticity jobs of its tiles (the heaviest: three transforms in and
+// three out) before those of r1 and r2.  Every XCD then gets the same mix of
+// heavy and light jobs, heaviest first (a field-major numbering put all the
+// vorticity jobs on the first three XCDs).
+__device__ __forceinline__ void col_job_map(int job, int ntiles, int& fld, int& tile) {
+  const int G = ntiles >= 8 ? ntiles / 8 : 1;
+  const int full = ntiles / G;
+  const int grp = job / (3 * G);
+  if (grp < full) {
+    const int r = job - grp * 3 * G;
+    fld = r / G;
+    tile = grp * G + r % G;
+  } else {   // the last, partial group: tiles full*G .. ntiles-1
+    const int g2 = ntiles - full * G;
+    const int r = job - full * 3 * G;
+    fld = r / g2;
+    tile = full * G + r % g2;
+  }
+}
+
 template <int MODE>
 __device__ void col_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq, bool pro) {
   const int tc = 4 * wave_nv(a.Ny);
   const int ntiles = (a.Nxh + tc - 1) / tc;
-  const int fld = job / ntiles, i0 = (job % ntiles) * tc;
+  int fld, tile;
+  col_job_map(job, ntiles, fld, tile);
+  const int i0 = tile * tc;
   switch (a.Ny) {
     case 16: col_job<16, MODE>(a, fld, i0, sm, twq, pro); break;
     case 32: col_job<32, MODE>(a, fld, i0, sm, twq, pro); break;
@@ -409,7 +434,9 @@ template <int NY, int MODE>
 __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const double2* twq, bool pro) {
   constexpr int tc = 4 * WaveLay<NY>::NV;
   const int ntiles = (a.Nxh + tc - 1) / tc;
-  col_job<NY, MODE>(a, job / ntiles, (job % ntiles) * tc, sm, twq, pro);
+  int fld, tile;
+  col_job_map(job, ntiles, fld, tile);
+  col_job<NY, MODE>(a, fld, tile * tc, sm, twq, pro);
 }
 
 // pro != 0: the prologue of the final-stage modes (S2, R4, E).
