@@ -147,7 +147,7 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
 // columns (ColTile), one column per lane group, FFTs along y per wavefront.
 template <int NY, int MODE>
 __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq,
-                                        bool pro_arg) {
+                                        double2* dbuf, bool pro_arg) {
   // prologue: final-stage kernels only (see is_prologue)
   const bool pro = (MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && pro_arg;
   constexpr int LPC = WaveLay<NY>::LPC, E = WaveLay<NY>::E;
@@ -163,9 +163,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   auto kyj = [&](int j) { return a.dky * (j < NY / 2 + 1 ? j : j - NY); };
   const int nyq_x = a.Nx / 2;
   bool bad = false;
-  auto stage_in = [&](const double2* __restrict__ src, double2 (&v)[16]) { T.stage_in(src, v); };
   auto stage_out = [&](double2* __restrict__ dst, const double2 (&v)[16]) { T.stage_out(dst, v); };
-  auto tile_load = [&](const double2* __restrict__ src) { T.load(src); };
   auto tile_store = [&](double2* __restrict__ dst) { T.store(dst); };
 
   double2 st[16];   // the state of the next stage (curr / new / old)
@@ -178,12 +176,26 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   } else {
     // ---- N_X from the forward set ------------------------------------------
     //   N_w = F0 + i ky F1 + ky^2 F2,  N_r = F_a + i ky F_b   (in st)
+    // Input tiles arrive by LDS DMA into dbuf, each issued as soon as the
+    // previous one has been read, so it loads while that one is transformed:
+    // F_a, F_b, (F2,) then the state tile of the stage update.
+    const TileDMA<NY> D{dbuf, a, i0};
+    const int cl = (threadIdx.x >> 6) * ColTile<NY>::CW + T.g;   // this lane's tile column
+    const bool from_old = MODE == MODE_S1 || MODE == MODE_R1 || MODE == MODE_E;
+    const double2* state_in = from_old ? a.old_[FLD] : a.new_[FLD];   // R2, R3: new first
     const int f0 = FLD == 0 ? 0 : (FLD == 1 ? 3 : 5);
-    stage_in(a.mix[f0], st);
+    D.issue(a.mix[f0]);
+    D.wait();
+    D.read(st, cl, lp);
+    __syncthreads();
+    D.issue(a.mix[f0 + 1]);
     wave_fft<NY, -1>(st, vb, lp, tw);
     {
       double2 v[16];
-      stage_in(a.mix[f0 + 1], v);
+      D.wait();
+      D.read(v, cl, lp);
+      __syncthreads();
+      D.issue(FLD == 0 ? a.mix[2] : state_in);
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -193,7 +205,10 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     }
     if (FLD == 0) {
       double2 v[16];
-      stage_in(a.mix[2], v);
+      D.wait();
+      D.read(v, cl, lp);
+      __syncthreads();
+      D.issue(state_in);
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -217,15 +232,14 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const int j = lp + LPC * m;
       if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
     }
+    D.wait();   // the state tile is in dbuf; updated values go to the wave buffers for the store
     if (MODE == MODE_S1 || MODE == MODE_S2 || MODE == MODE_R1 || MODE == MODE_R4 || MODE == MODE_E) {
       // one state tile: S1 / R1 / E read old, S2 / R4 read new
-      const bool from_old = MODE == MODE_S1 || MODE == MODE_R1 || MODE == MODE_E;
-      tile_load(from_old ? a.old_[FLD] : a.new_[FLD]);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
-        const double2 in = *sl;
+        const double2 in = D.at(j, cl);
         const double2 n = st[m];
         const double Lk = lin(j);
         const double IF = exp(Lk * a.dt);
@@ -267,23 +281,23 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       tile_store((MODE == MODE_S1 || MODE == MODE_R1) ? a.new_[FLD] : a.old_[FLD]);
     } else {
       // R2, R3: new += 1/3 IFh dt N (new tile), then curr from old (old tile)
-      tile_load(a.new_[FLD]);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
-        const double2 nw = *sl;
+        const double2 nw = D.at(j, cl);
         const double IFh = exp(lin(j) * a.dt * 0.5);
         const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
         const double h = 1.0 / 3.0 * IFh;                                          // RK4.cuh:37-61
         *sl = make_double2(nw.x + h * bn.x, nw.y + h * bn.y);
       }
-      tile_store(a.new_[FLD]);
-      tile_load(a.old_[FLD]);
+      tile_store(a.new_[FLD]);   // its barriers: every wave has read dbuf
+      D.issue(a.old_[FLD]);
+      D.wait();
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
-        const double2 old = vb[swz(j)];
+        const double2 old = D.at(j, cl);
         const double Lk = lin(j);
         const double IFh = exp(Lk * a.dt * 0.5);
         const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
@@ -395,20 +409,20 @@ __device__ __forceinline__ void col_job_map(int job, int ntiles, int& fld, int& 
 }
 
 template <int MODE>
-__device__ void col_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq, bool pro) {
+__device__ void col_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq, double2* dbuf, bool pro) {
   const int tc = 4 * wave_nv(a.Ny);
   const int ntiles = (a.Nxh + tc - 1) / tc;
   int fld, tile;
   col_job_map(job, ntiles, fld, tile);
   const int i0 = tile * tc;
   switch (a.Ny) {
-    case 16: col_job<16, MODE>(a, fld, i0, sm, twq, pro); break;
-    case 32: col_job<32, MODE>(a, fld, i0, sm, twq, pro); break;
-    case 64: col_job<64, MODE>(a, fld, i0, sm, twq, pro); break;
-    case 128: col_job<128, MODE>(a, fld, i0, sm, twq, pro); break;
-    case 256: col_job<256, MODE>(a, fld, i0, sm, twq, pro); break;
-    case 512: col_job<512, MODE>(a, fld, i0, sm, twq, pro); break;
-    default: col_job<1024, MODE>(a, fld, i0, sm, twq, pro); break;
+    case 16: col_job<16, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
+    case 32: col_job<32, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
+    case 64: col_job<64, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
+    case 128: col_job<128, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
+    case 256: col_job<256, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
+    case 512: col_job<512, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
+    default: col_job<1024, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
   }
 }
 
@@ -431,12 +445,13 @@ __global__ __launch_bounds__(384) void k_row(const RunDesc* __restrict__ runs, c
 }
 
 template <int NY, int MODE>
-__device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const double2* twq, bool pro) {
+__device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const double2* twq,
+                                               double2* dbuf, bool pro) {
   constexpr int tc = 4 * WaveLay<NY>::NV;
   const int ntiles = (a.Nxh + tc - 1) / tc;
   int fld, tile;
   col_job_map(job, ntiles, fld, tile);
-  col_job<NY, MODE>(a, fld, tile * tc, sm, twq, pro);
+  col_job<NY, MODE>(a, fld, tile * tc, sm, twq, dbuf, pro);
 }
 
 // pro != 0: the prologue of the final-stage modes (S2, R4, E).
@@ -445,13 +460,14 @@ __global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, c
                                              int nwork, const double2* __restrict__ tw1024, int pro) {
   __shared__ double2 sm[4 * (kBuf + 1)];
   __shared__ double2 twq[kTwq];
+  __shared__ double2 dbuf[4096];   // DMA tile buffer (TileDMA), 256 * E entries used
   stage_twiddles(twq, tw1024, threadIdx.x, 256);
   __syncthreads();
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   for (int w = L; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
-    if constexpr (NFIX > 0) col_dispatch_n<NFIX, MODE>(runs[it.x], it.y, sm, twq, pro != 0);
-    else col_dispatch<MODE>(runs[it.x], it.y, sm, twq, pro != 0);
+    if constexpr (NFIX > 0) col_dispatch_n<NFIX, MODE>(runs[it.x], it.y, sm, twq, dbuf, pro != 0);
+    else col_dispatch<MODE>(runs[it.x], it.y, sm, twq, dbuf, pro != 0);
     __syncthreads();
   }
 }

@@ -213,5 +213,58 @@ struct ColTile {
   }
 };
 
+// Column tile of a column job in a separate LDS buffer D (row-major
+// [row][TC], TC * NY = 256 * E entries) filled by LDS DMA
+// (global_load_lds_dwordx4): the loads need no registers and complete in the
+// background, so a job issues its next input tile right after reading the
+// current one and runs the FFT meanwhile.  Instruction q of wave w writes the
+// 64 entries (w*E + q)*64 + lane: whole row segments, as ColTile::load reads
+// them.  Columns past the last one (i >= Nxh) repeat it; their results are
+// never stored.
+template <int NY>
+struct TileDMA {
+  static constexpr int CW = WaveLay<NY>::NV, TC = 4 * CW, LPC = WaveLay<NY>::LPC, E = WaveLay<NY>::E;
+  double2* D;
+  const RunDesc& a;
+  int i0;
+  __device__ __forceinline__ void issue(const double2* __restrict__ src) const {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if constexpr (64 % TC == 0) {
+      // a lane keeps its tile column c = lane % TC and moves 64/TC rows per
+      // instruction: one base offset plus a uniform stride
+      const int c = lane % TC;
+      const int ii = i0 + c < a.Nxh ? i0 + c : a.Nxh - 1;
+      const unsigned base = (unsigned)((wave * E * (64 / TC) + lane / TC) * a.pitch + ii);
+      const unsigned stride = (unsigned)((64 / TC) * a.pitch);
+#pragma unroll
+      for (int q = 0; q < E; ++q)
+        __builtin_amdgcn_global_load_lds((const QS_GLOBAL void*)(src + (base + q * stride)),
+                                         (__attribute__((address_space(3))) void*)(D + (wave * E + q) * 64), 16, 0,
+                                         0);
+    } else {   // tiles wider than a wave (NY <= 32): 64 consecutive columns of one row each
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int e = (wave * E + q) * 64 + lane;
+        const int c = e % TC, j = e / TC;
+        const int ii = i0 + c < a.Nxh ? i0 + c : a.Nxh - 1;
+        __builtin_amdgcn_global_load_lds((const QS_GLOBAL void*)(src + (unsigned)(j * a.pitch + ii)),
+                                         (__attribute__((address_space(3))) void*)(D + (wave * E + q) * 64), 16, 0,
+                                         0);
+      }
+    }
+  }
+  // this wave's transfers done (vmcnt 0), then every wave's
+  __device__ __forceinline__ void wait() const {
+    __builtin_amdgcn_s_waitcnt(0x0f70);
+    __syncthreads();
+  }
+  // tile column c, rows lp + LPC*m, in the FFT lane layout
+  __device__ __forceinline__ void read(double2 (&v)[16], int c, int lp) const {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = D[(lp + LPC * m) * TC + c];
+  }
+  __device__ __forceinline__ double2 at(int j, int c) const { return D[j * TC + c]; }
+};
+
 }  // namespace
 }  // namespace qs
