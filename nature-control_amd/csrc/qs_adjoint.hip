@@ -234,7 +234,8 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
 // Column adjoint job: one field on a tile of columns (as the forward).
 // ===========================================================================
 template <int NY, int AMODE>
-__device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq) {
+__device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq,
+                                         double2* dbuf) {
   constexpr int LPC = NY / 16;
   const TwN<NY> tw{twq};
   ColTile<NY> T(a, sm, i0);
@@ -284,12 +285,24 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     T.store(a.lam_[FLD]);
   } else {
     // ---- cbar = (y inverse set)^T Mbar: y FFTs of the field's Mbar arrays ----
+    // Input tiles by LDS DMA (TileDMA), each issued while the previous one is
+    // transformed; the last one is the state tile of the combination below.
+    const TileDMA<NY> D{dbuf, a, i0};
+    const int cl = wave * ColTile<NY>::CW + T.g;   // this lane's tile column
+    const double2* state_in = AMODE == AMODE_B ? a.lam_[FLD] : a.acc_[FLD];
     if (FLD == 0) {
-      T.stage_in(a.mix[0], st);
+      D.issue(a.mix[0]);
+      D.wait();
+      D.read(st, cl, lp);
+      __syncthreads();
+      D.issue(a.mix[1]);
       wave_fft<NY, -1>(st, vb, lp, tw);
       {
         double2 v[16];
-        T.stage_in(a.mix[1], v);
+        D.wait();
+        D.read(v, cl, lp);
+        __syncthreads();
+        D.issue(a.mix[2]);
         wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {   // psi = W / k^2
@@ -302,7 +315,10 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       }
       {
         double2 v[16];
-        T.stage_in(a.mix[2], v);
+        D.wait();
+        D.read(v, cl, lp);
+        __syncthreads();
+        D.issue(state_in);
         wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {   // i ky psi -> conj: -i ky / k^2
@@ -316,10 +332,17 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       if (i0 == 0 && threadIdx.x == 0) st[0].x += w00sum;   // element (i = 0, j = 0)
     } else {
       const int m0 = FLD == 1 ? 3 : 5;
-      T.stage_in(a.mix[m0], st);
+      D.issue(a.mix[m0]);
+      D.wait();
+      D.read(st, cl, lp);
+      __syncthreads();
+      D.issue(a.mix[m0 + 1]);
       wave_fft<NY, -1>(st, vb, lp, tw);
       double2 v[16];
-      T.stage_in(a.mix[m0 + 1], v);
+      D.wait();
+      D.read(v, cl, lp);
+      __syncthreads();
+      D.issue(state_in);
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
@@ -328,36 +351,38 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       }
     }
     // ---- IFRK2 reverse combinations ----------------------------------------
+    // (the state tile, lambda or acc, is in dbuf; results go to the wave
+    // buffers for the store)
+    D.wait();
     if (AMODE == AMODE_B) {
-      T.load(a.lam_[FLD]);
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
-        const double2 lamv = *sl;
+        const double2 lamv = D.at(j, cl);
         const double E = IFof(j);
         *sl = make_double2(E * (lamv.x + st[m].x), E * (lamv.y + st[m].y));
         st[m] = make_double2(E * (0.5 * dt * lamv.x + dt * st[m].x), E * (0.5 * dt * lamv.y + dt * st[m].y));
       }
       T.store(a.acc_[FLD]);
     } else if (AMODE == AMODE_A) {
-      T.load(a.acc_[FLD]);
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
         const double c = keep(j) ? 1.0 : 0.0;
-        const double2 lamv = make_double2(c * (sl->x + st[m].x), c * (sl->y + st[m].y));
+        const double2 acc = D.at(j, cl);
+        const double2 lamv = make_double2(c * (acc.x + st[m].x), c * (acc.y + st[m].y));
         *sl = lamv;
         st[m] = make_double2(0.5 * dt * lamv.x, 0.5 * dt * lamv.y);
       }
       T.store(a.lam_[FLD]);
     } else {   // AMODE_END
-      T.load(a.acc_[FLD]);
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
-        double2* sl = vb + swz(lp + LPC * m);
-        *sl = make_double2(sl->x + st[m].x, sl->y + st[m].y);
+        const int j = lp + LPC * m;
+        const double2 acc = D.at(j, cl);
+        vb[swz(j)] = make_double2(acc.x + st[m].x, acc.y + st[m].y);
       }
       if (a.uout[FLD]) T.store(a.uout[FLD]);
       return;
@@ -439,6 +464,7 @@ __global__ __launch_bounds__(256) void k_acol(const RunDesc* __restrict__ runs, 
                                               int nwork, const double2* __restrict__ tw1024) {
   __shared__ double2 sm[4 * (kBuf + 1)];
   __shared__ double2 twq[kTwq];
+  __shared__ double2 dbuf[4096];   // DMA tile buffer (TileDMA)
   stage_twiddles(twq, tw1024, threadIdx.x, 256);
   __syncthreads();
   const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -447,7 +473,9 @@ __global__ __launch_bounds__(256) void k_acol(const RunDesc* __restrict__ runs, 
     const int2 it = work[w];
     const RunDesc& a = runs[it.x];
     const int ntiles = (a.Nxh + tc - 1) / tc;
-    acol_job<NY, AMODE>(a, it.y / ntiles, (it.y % ntiles) * tc, sm, twq);
+    int fld, tile;
+    col_job_map(it.y, ntiles, fld, tile);   // XCD-balanced numbering, as the forward column kernels
+    acol_job<NY, AMODE>(a, fld, tile * tc, sm, twq, dbuf);
     __syncthreads();
   }
 }

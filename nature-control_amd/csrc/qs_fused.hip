@@ -385,29 +385,6 @@ __device__ void row_dispatch(const RunDesc& a, int job, double2* sm, const doubl
   }
 }
 
-// Column job index -> (field, tile).  The column kernels remap block ids so
-// that each XCD takes a contiguous range of about 3 * ntiles / 8 jobs; jobs
-// are therefore numbered in groups of G = ntiles / 8 tiles, each group listing
-// the vorticity jobs of its tiles (the heaviest: three transforms in and
-// three out) before those of r1 and r2.  Every XCD then gets the same mix of
-// heavy and light jobs, heaviest first (a field-major numbering put all the
-// vorticity jobs on the first three XCDs).
-__device__ __forceinline__ void col_job_map(int job, int ntiles, int& fld, int& tile) {
-  const int G = ntiles >= 8 ? ntiles / 8 : 1;
-  const int full = ntiles / G;
-  const int grp = job / (3 * G);
-  if (grp < full) {
-    const int r = job - grp * 3 * G;
-    fld = r / G;
-    tile = grp * G + r % G;
-  } else {   // the last, partial group: tiles full*G .. ntiles-1
-    const int g2 = ntiles - full * G;
-    const int r = job - full * 3 * G;
-    fld = r / g2;
-    tile = full * G + r % g2;
-  }
-}
-
 template <int MODE>
 __device__ void col_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq, double2* dbuf, bool pro) {
   const int tc = 4 * wave_nv(a.Ny);

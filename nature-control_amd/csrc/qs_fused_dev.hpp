@@ -213,6 +213,29 @@ struct ColTile {
   }
 };
 
+// Column job index -> (field, tile).  The column kernels remap block ids so
+// that each XCD takes a contiguous range of about 3 * ntiles / 8 jobs; jobs
+// are therefore numbered in groups of G = ntiles / 8 tiles, each group listing
+// the vorticity jobs of its tiles (the heaviest: three transforms in and
+// three out) before those of r1 and r2.  Every XCD then gets the same mix of
+// heavy and light jobs, heaviest first (a field-major numbering put all the
+// vorticity jobs on the first three XCDs).
+__device__ __forceinline__ void col_job_map(int job, int ntiles, int& fld, int& tile) {
+  const int G = ntiles >= 8 ? ntiles / 8 : 1;
+  const int full = ntiles / G;
+  const int grp = job / (3 * G);
+  if (grp < full) {
+    const int r = job - grp * 3 * G;
+    fld = r / G;
+    tile = grp * G + r % G;
+  } else {   // the last, partial group: tiles full*G .. ntiles-1
+    const int g2 = ntiles - full * G;
+    const int r = job - full * 3 * G;
+    fld = r / g2;
+    tile = full * G + r % g2;
+  }
+}
+
 // Column tile of a column job in a separate LDS buffer D (row-major
 // [row][TC], TC * NY = 256 * E entries) filled by LDS DMA
 // (global_load_lds_dwordx4): the loads need no registers and complete in the
