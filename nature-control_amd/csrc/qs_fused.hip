@@ -8,11 +8,11 @@
 // first pass in registers, then LDS Stockham passes ordered by wave-level
 // synchronisation only):
 //
-//  row job  (6 waves, one job = 1024/N rows):
+//  row job  (6 waves, one job = NV rows: 1024/N for N = 2^a, 768/N for 3 * 2^a):
 //    7 mixed arrays (x-spectral, y-physical) -> five two-for-one x C2R
 //    (waves 0-4) -> nine physical fields in LDS -> pointwise products in place
 //    -> six two-for-one x R2C (waves 0-5) -> 7 mixed arrays.
-//  column job (4 waves, one job = one field x 4*1024/Ny columns):
+//  column job (4 waves, one job = one field x 4 NV columns):
 //    y FFT of the field's forward arrays -> N_X -> IFRK2 stage update of the
 //    state -> the next stage's derived spectra -> y inverse FFT -> mixed arrays.
 //    Stage 1 reads old and writes new (IF old + dt IF N/2); stage 2 reads new
@@ -36,7 +36,7 @@
 namespace qs {
 namespace {
 
-// Row job layout (one job = NV = 1024/N rows, 6 waves, 6 LDS buffers):
+// Row job layout (one job = NV = WaveLay<N>::NV rows, 6 waves, 6 LDS buffers):
 //  1. the 7 mixed rows are loaded cooperatively (coalesced) into a staging
 //     view of the buffers;
 //  2. waves 0-4 build their two-for-one C2R inputs from the staging area,
@@ -143,7 +143,7 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
 // ===========================================================================
 // Column job
 // ===========================================================================
-// Column job: one field (FLD 0 = w, 1 = r1, 2 = r2) on a tile of 4*1024/NY
+// Column job: one field (FLD 0 = w, 1 = r1, 2 = r2) on a tile of 4 NV
 // columns (ColTile), one column per lane group, FFTs along y per wavefront.
 template <int NY, int MODE>
 __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq,
@@ -517,7 +517,7 @@ int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, 
 }  // namespace qs
 
 // ---------------------------------------------------------------------------
-// Test hook: per-wavefront FFT of CW = 1024/N vectors (one wave, one block).
+// Test hook: per-wavefront FFT of the CW = WaveLay<N>::NV vectors of one wave.
 // in/out: CW vectors of length N, contiguous.
 // ---------------------------------------------------------------------------
 namespace qs {

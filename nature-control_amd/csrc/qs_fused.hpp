@@ -46,7 +46,7 @@ struct RunDesc {
   double2* scratch;               // row adjoint: [grid][6][1024] physical product adjoints
 };
 
-// Jobs of one run per launch (row: 1024/Nx rows each; column: 3 fields x tiles).
+// Jobs of one run per launch (row: WaveLay<Nx>::NV rows each; column: 3 fields x tiles).
 int row_jobs(const RunDesc& a);
 int col_jobs(const RunDesc& a);
 
