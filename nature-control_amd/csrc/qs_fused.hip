@@ -47,15 +47,14 @@ namespace {
 //     (A, B), (-u r1, -v r1), (-u r2, -v r2), (P11, P12), (P21, -), (loc1, loc2);
 //  5. forward x FFTs (per wave), spectra back into the buffers;
 //  6. cooperative unpack + spectral combination -> F0..F6 (coalesced stores).
-template <int N>
-__device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm, const double2* twq) {
+template <int N, class TW>
+__device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm, const TW& tw) {
   using L = WaveLay<N>;
   constexpr int NV = L::NV;             // rows of this job (= vectors per wave FFT)
   constexpr int LPC = L::LPC, E = L::E;
   constexpr int VS = L::VS;             // padded vector stride inside a wave buffer
   constexpr int H = N / 2 + 1;
   constexpr int RH = NV * H;            // staged bins per array
-  const TwN<N> tw{twq};
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;     // lane = lp * NV + g
@@ -145,13 +144,12 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
 // ===========================================================================
 // Column job: one field (FLD 0 = w, 1 = r1, 2 = r2) on a tile of 4 NV
 // columns (ColTile), one column per lane group, FFTs along y per wavefront.
-template <int NY, int MODE>
-__device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq,
+template <int NY, int MODE, class TW>
+__device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const TW& tw,
                                         double2* dbuf, bool pro_arg) {
   // prologue: final-stage kernels only (see is_prologue)
   const bool pro = (MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && pro_arg;
   constexpr int LPC = WaveLay<NY>::LPC, E = WaveLay<NY>::E;
-  const TwN<NY> tw{twq};
   ColTile<NY> T(a, sm, i0);
   const int lp = T.lp;
   const int i = i0 + (threadIdx.x >> 6) * ColTile<NY>::CW + T.g;
@@ -368,20 +366,20 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
 // Kernels: one run (grid over jobs) and an ensemble (work list over runs).
 // ===========================================================================
 
-template <int N>
-__device__ __forceinline__ void row_dispatch_n(const RunDesc& a, int job, double2* sm, const double2* twq) {
-  row_job<N>(a, job * WaveLay<N>::NV, sm, twq);
+template <int N, class TW>
+__device__ __forceinline__ void row_dispatch_n(const RunDesc& a, int job, double2* sm, const TW& tw) {
+  row_job<N>(a, job * WaveLay<N>::NV, sm, tw);
 }
 
 __device__ void row_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq) {
   switch (a.Nx) {
-    case 16: row_dispatch_n<16>(a, job, sm, twq); break;
-    case 32: row_dispatch_n<32>(a, job, sm, twq); break;
-    case 64: row_dispatch_n<64>(a, job, sm, twq); break;
-    case 128: row_dispatch_n<128>(a, job, sm, twq); break;
-    case 256: row_dispatch_n<256>(a, job, sm, twq); break;
-    case 512: row_dispatch_n<512>(a, job, sm, twq); break;
-    default: row_dispatch_n<1024>(a, job, sm, twq); break;
+    case 16: row_dispatch_n<16>(a, job, sm, TwN<16>{twq}); break;
+    case 32: row_dispatch_n<32>(a, job, sm, TwN<32>{twq}); break;
+    case 64: row_dispatch_n<64>(a, job, sm, TwN<64>{twq}); break;
+    case 128: row_dispatch_n<128>(a, job, sm, TwN<128>{twq}); break;
+    case 256: row_dispatch_n<256>(a, job, sm, TwN<256>{twq}); break;
+    case 512: row_dispatch_n<512>(a, job, sm, TwN<512>{twq}); break;
+    default: row_dispatch_n<1024>(a, job, sm, TwN<1024>{twq}); break;
   }
 }
 
@@ -393,13 +391,13 @@ __device__ void col_dispatch(const RunDesc& a, int job, double2* sm, const doubl
   col_job_map(job, ntiles, fld, tile);
   const int i0 = tile * tc;
   switch (a.Ny) {
-    case 16: col_job<16, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
-    case 32: col_job<32, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
-    case 64: col_job<64, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
-    case 128: col_job<128, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
-    case 256: col_job<256, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
-    case 512: col_job<512, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
-    default: col_job<1024, MODE>(a, fld, i0, sm, twq, dbuf, pro); break;
+    case 16: col_job<16, MODE>(a, fld, i0, sm, TwN<16>{twq}, dbuf, pro); break;
+    case 32: col_job<32, MODE>(a, fld, i0, sm, TwN<32>{twq}, dbuf, pro); break;
+    case 64: col_job<64, MODE>(a, fld, i0, sm, TwN<64>{twq}, dbuf, pro); break;
+    case 128: col_job<128, MODE>(a, fld, i0, sm, TwN<128>{twq}, dbuf, pro); break;
+    case 256: col_job<256, MODE>(a, fld, i0, sm, TwN<256>{twq}, dbuf, pro); break;
+    case 512: col_job<512, MODE>(a, fld, i0, sm, TwN<512>{twq}, dbuf, pro); break;
+    default: col_job<1024, MODE>(a, fld, i0, sm, TwN<1024>{twq}, dbuf, pro); break;
   }
 }
 
@@ -410,25 +408,30 @@ __global__ __launch_bounds__(384) void k_row(const RunDesc* __restrict__ runs, c
                                              int nwork, const double2* __restrict__ tw1024) {
   __shared__ double2 sm[6 * kBuf];
   __shared__ double2 twq[kTwq];
+  __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
   stage_twiddles(twq, tw1024, threadIdx.x, 384);
   __syncthreads();
+  if constexpr (use_twt<NFIX>()) {
+    build_twtab<NFIX>(twt, twq, threadIdx.x, 384);
+    __syncthreads();
+  }
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   for (int w = L; w < nwork; w += gridDim.x) {
     const int2 it = work[w];          // (run, job)
-    if constexpr (NFIX > 0) row_dispatch_n<NFIX>(runs[it.x], it.y, sm, twq);
+    if constexpr (NFIX > 0) row_dispatch_n<NFIX>(runs[it.x], it.y, sm, fused_tw<NFIX>(twq, twt));
     else row_dispatch(runs[it.x], it.y, sm, twq);
     __syncthreads();
   }
 }
 
-template <int NY, int MODE>
-__device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const double2* twq,
+template <int NY, int MODE, class TW>
+__device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const TW& tw,
                                                double2* dbuf, bool pro) {
   constexpr int tc = 4 * WaveLay<NY>::NV;
   const int ntiles = (a.Nxh + tc - 1) / tc;
   int fld, tile;
   col_job_map(job, ntiles, fld, tile);
-  col_job<NY, MODE>(a, fld, tile * tc, sm, twq, dbuf, pro);
+  col_job<NY, MODE>(a, fld, tile * tc, sm, tw, dbuf, pro);
 }
 
 // pro != 0: the prologue of the final-stage modes (S2, R4, E).
@@ -438,12 +441,17 @@ __global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, c
   __shared__ double2 sm[4 * (kBuf + 1)];
   __shared__ double2 twq[kTwq];
   __shared__ double2 dbuf[4096];   // DMA tile buffer (TileDMA), 256 * E entries used
+  __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
   stage_twiddles(twq, tw1024, threadIdx.x, 256);
   __syncthreads();
+  if constexpr (use_twt<NFIX>()) {
+    build_twtab<NFIX>(twt, twq, threadIdx.x, 256);
+    __syncthreads();
+  }
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   for (int w = L; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
-    if constexpr (NFIX > 0) col_dispatch_n<NFIX, MODE>(runs[it.x], it.y, sm, twq, dbuf, pro != 0);
+    if constexpr (NFIX > 0) col_dispatch_n<NFIX, MODE>(runs[it.x], it.y, sm, fused_tw<NFIX>(twq, twt), dbuf, pro != 0);
     else col_dispatch<MODE>(runs[it.x], it.y, sm, twq, dbuf, pro != 0);
     __syncthreads();
   }

@@ -14,12 +14,74 @@ constexpr int kTwq = 256;            // staged twiddles: the quarter table of W_
 // for N = 3 * 2^a <= 768 (the kernel is handed the matching table).
 template <int N>
 struct TwN {
+  static constexpr bool kTable = false;
   const double2* p;
   __device__ __forceinline__ double2 operator()(int m) const {
     if constexpr (N % 3 == 0) return TwQuarter<768>{p}(m * (768 / N));
     else return TwQuarter<1024>{p}(m * (1024 / N));
   }
 };
+
+// Per-pass twiddle tables of the per-wave FFT of length N: for each LDS pass
+// (P = E, E R_1, ...; radix R = min(RMAX, N/P)) the factors W_N^(r k STEP),
+// STEP = N/(P R), at twtab_off(P) + k (R-1) + r - 1, for k < P, 1 <= r < R.
+// Built once per block from the quarter table (build_twtab), they replace the
+// per-butterfly quadrant selects and power products: one LDS read per factor.
+template <int N>
+constexpr int twtab_radix(int P) { return N / P >= WaveLay<N>::RMAX ? WaveLay<N>::RMAX : N / P; }
+template <int N>
+constexpr int twtab_off(int P) {
+  int s = 0;
+  for (int Q = WaveLay<N>::E; Q < P; Q *= twtab_radix<N>(Q)) s += Q * (twtab_radix<N>(Q) - 1);
+  return s;
+}
+template <int N>
+constexpr int twtab_size() { return twtab_off<N>(N); }
+static_assert(twtab_size<768>() == 756, "768: 12*3 + 48*3 + 192*3");
+template <int N>
+struct TwT {
+  static constexpr bool kTable = true;
+  const double2* q;   // staged quarter table
+  const double2* t;   // the pass tables
+  __device__ __forceinline__ double2 operator()(int m) const { return TwN<N>{q}(m); }
+  template <int P>
+  __device__ __forceinline__ const double2* pass(int k) const {
+    return t + twtab_off<N>(P) + k * (twtab_radix<N>(P) - 1);
+  }
+};
+
+// Fills the pass tables from the staged quarter table q (which must be
+// complete: a barrier between the two); the caller synchronises after.
+template <int N>
+__device__ __forceinline__ void build_twtab(double2* t, const double2* q, int tid, int nt) {
+  const TwN<N> tw{q};
+  for (int P = WaveLay<N>::E; P < N; P *= twtab_radix<N>(P)) {
+    const int R = twtab_radix<N>(P), STEP = N / (P * R), base = twtab_off<N>(P);
+    for (int e = tid; e < P * (R - 1); e += nt) {
+      const int k = e / (R - 1), r = e % (R - 1) + 1;
+      t[base + e] = tw(r * k * STEP);
+    }
+  }
+}
+
+// The fused kernels of fixed size NFIX use the tables for N = 3 * 2^a, whose
+// radix-4 pass chain is long (768: 4% faster per step); for N = 2^a the
+// computed factors are as fast or faster (1024: the row kernel 2.5 us slower
+// with tables), and NFIX = 0 (per-run sizes) has no single table.
+template <int NFIX>
+constexpr bool use_twt() { return NFIX > 0 && NFIX % 3 == 0; }
+// LDS entries such a kernel reserves for its tables (1 when it has none).
+template <int NFIX>
+constexpr int twt_len() {
+  if constexpr (use_twt<NFIX>()) return twtab_size<NFIX>();
+  else return 1;
+}
+// The twiddle accessor of a fixed-size fused kernel (tables built if used).
+template <int NFIX>
+__device__ __forceinline__ auto fused_tw(const double2* twq, const double2* twt) {
+  if constexpr (use_twt<NFIX>()) return TwT<NFIX>{twq, twt};
+  else return TwN<NFIX>{twq};
+}
 
 // ===========================================================================
 // Row job

@@ -454,26 +454,38 @@ __device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, in
     const int k = (int)((unsigned)b % (unsigned)P);   // b >= 0: an AND for P = 2^a
 #pragma unroll
     for (int r = 0; r < R; ++r) t[q][r] = buf[swz(b + r * Q)];
-    // Twiddles w^r, w = W^(k STEP): the powers w^1, w^2, w^4, w^8 come from the
-    // table, every other w^r is the product of the powers of its set bits (at
-    // most three extra products, a few ulp): 4 table reads per butterfly
+    // Twiddles w^r, w = W^(k STEP).  With a per-pass table (TwT): one LDS read
+    // each.  Otherwise the powers w^1, w^2, w^4, w^8 come from the quarter
+    // table and every other w^r is the product of the powers of its set bits
+    // (at most three extra products, a few ulp): 4 table reads per butterfly
     // instead of R - 1, and only 4 twiddles live in registers.
-    constexpr int NPOW = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : 1));
-    double2 wp[NPOW];
+    if constexpr (TW::kTable) {
+      // per-pass table: W^(r k STEP) for r = 1 .. R-1 at tw.pass<P>(k)
+      const double2* tp = tw.template pass<P>(k);
 #pragma unroll
-    for (int e = 0; e < NPOW; ++e) {
-      double2 w = tw((1 << e) * k * STEP);
-      if (SIGN > 0) w.y = -w.y;
-      wp[e] = w;
-    }
+      for (int r = 1; r < R; ++r) {
+        double2 w = tp[r - 1];
+        if (SIGN > 0) w.y = -w.y;
+        t[q][r] = cmul(t[q][r], w);
+      }
+    } else {
+      constexpr int NPOW = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : 1));
+      double2 wp[NPOW];
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-      const int lo = __builtin_ctz(r);
-      double2 w = wp[lo];
+      for (int e = 0; e < NPOW; ++e) {
+        double2 w = tw((1 << e) * k * STEP);
+        if (SIGN > 0) w.y = -w.y;
+        wp[e] = w;
+      }
 #pragma unroll
-      for (int e = lo + 1; e < NPOW; ++e)
-        if (r & (1 << e)) w = cmul(w, wp[e]);
-      t[q][r] = cmul(t[q][r], w);
+      for (int r = 1; r < R; ++r) {
+        const int lo = __builtin_ctz(r);
+        double2 w = wp[lo];
+#pragma unroll
+        for (int e = lo + 1; e < NPOW; ++e)
+          if (r & (1 << e)) w = cmul(w, wp[e]);
+        t[q][r] = cmul(t[q][r], w);
+      }
     }
     dft_any<R, SIGN>(t[q]);
   }
