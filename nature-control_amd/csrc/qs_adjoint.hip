@@ -108,7 +108,7 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
 
   // ---- B. the stage's physical fields (buffers 0-4) -------------------------
   stage_rows<N>(a.minp, a, row0, sm);   // its barrier also publishes the scratch writes
-  inverse_fields<N>(a, row0, sm, tw, v);
+  inverse_fields<N>(a, row0, sm, tw, v, gload(a.w00));
 
   // ---- C. pointwise Jacobian transposed (friction/Qsolver.cuh:63-300) -------
   const double alu = a.alpha_p ? gload(a.alpha_p) : a.alpha;

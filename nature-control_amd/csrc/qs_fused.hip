@@ -47,8 +47,14 @@ namespace {
 //     (A, B), (-u r1, -v r1), (-u r2, -v r2), (P11, P12), (P21, -), (loc1, loc2);
 //  5. forward x FFTs (per wave), spectra back into the buffers;
 //  6. cooperative unpack + spectral combination -> F0..F6 (coalesced stores).
+// The part of a row job after staging: phases 3-6, run by threads 0..383.
+// w00 = *a.w00; alu: the step's uniform activity (*a.alpha_p, or a.alpha).
+// Block barriers: kRowBodyBarriers (two in inverse_fields, one after the
+// products, one after the forward FFTs).
+constexpr int kRowBodyBarriers = 4;
 template <int N, class TW>
-__device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm, const TW& tw) {
+__device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm, const TW& tw, double w00,
+                                         double alu) {
   using L = WaveLay<N>;
   constexpr int NV = L::NV;             // rows of this job (= vectors per wave FFT)
   constexpr int LPC = L::LPC, E = L::E;
@@ -60,12 +66,10 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
   const int lp = lane / NV, g = lane % NV;     // lane = lp * NV + g
   double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
 
-  stage_rows<N>(a.minp, a, row0, sm);
   double2 v[16];
-  inverse_fields<N>(a, row0, sm, tw, v);
+  inverse_fields<N>(a, row0, sm, tw, v, w00);
 
   // ---- 4. pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) --
-  const double alu = a.alpha_p ? gload(a.alpha_p) : a.alpha;   // uniform activity of this step
   for (int p = tid; p < NV * N; p += 384) {
     const int gg = p / N, x = p % N;
     const int o = gg * VS + swz(x);
@@ -137,6 +141,12 @@ __device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm,
     gstore(a.mix[5] + ro, make_double2(Lc2.x - kx * U2.y, Lc2.y + kx * U2.x));               // loc2 + i kx U2
     gstore(a.mix[6] + ro, V2);
   }
+}
+
+template <int N, class TW>
+__device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm, const TW& tw) {
+  stage_rows<N>(a.minp, a, row0, sm);
+  row_body<N>(a, row0, sm, tw, gload(a.w00), a.alpha_p ? gload(a.alpha_p) : a.alpha);
 }
 
 // ===========================================================================
@@ -401,26 +411,66 @@ __device__ void col_dispatch(const RunDesc& a, int job, double2* sm, const doubl
   }
 }
 
-// NFIX > 0: every run of the work list has Nx = Ny = NFIX (one instantiation);
-// NFIX = 0: per-run dispatch on the mesh size (ensembles of uneven runs).
+// NFIX > 0: every run of the work list has Nx = Ny = NFIX (one instantiation),
+// six compute waves and two loader waves (RowLoader) per block;
+// NFIX = 0: per-run dispatch on the mesh size (ensembles of uneven runs), six
+// waves that stage their own rows.
 template <int NFIX>
-__global__ __launch_bounds__(384) void k_row(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
-                                             int nwork, const double2* __restrict__ tw1024) {
+constexpr int row_threads() { return NFIX > 0 ? 512 : 384; }
+
+template <int NFIX>
+__global__ __launch_bounds__(row_threads<NFIX>()) void k_row(const RunDesc* __restrict__ runs,
+                                                             const int2* __restrict__ work, int nwork,
+                                                             const double2* __restrict__ tw1024) {
+  constexpr int NT = row_threads<NFIX>();
   __shared__ double2 sm[6 * kBuf];
   __shared__ double2 twq[kTwq];
   __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
-  stage_twiddles(twq, tw1024, threadIdx.x, 384);
+  __shared__ double rsc[2];                  // the job's (w00, alu), from the loader
+  stage_twiddles(twq, tw1024, threadIdx.x, NT);
   __syncthreads();
   if constexpr (use_twt<NFIX>()) {
-    build_twtab<NFIX>(twt, twq, threadIdx.x, 384);
+    build_twtab<NFIX>(twt, twq, threadIdx.x, NT);
     __syncthreads();
   }
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  for (int w = L; w < nwork; w += gridDim.x) {
-    const int2 it = work[w];          // (run, job)
-    if constexpr (NFIX > 0) row_dispatch_n<NFIX>(runs[it.x], it.y, sm, fused_tw<NFIX>(twq, twt));
-    else row_dispatch(runs[it.x], it.y, sm, twq);
-    __syncthreads();
+  if constexpr (NFIX > 0) {
+    constexpr int NV = WaveLay<NFIX>::NV;
+    if (threadIdx.x >= 384) {
+      // Loader waves.  Per job they pass exactly the compute waves' block
+      // barriers: staging ready, the kRowBodyBarriers of row_body, job done.
+      const int lt = threadIdx.x - 384;
+      RowLoader<NFIX> rl;
+      if (L < nwork) {
+        const int2 it = work[L];
+        rl.issue(runs[it.x], it.y * NV, lt);
+      }
+      for (int w = L; w < nwork; w += gridDim.x) {
+        rl.commit(sm, rsc, lt);            // this job's rows (waits for the loads)
+        const int wn = w + gridDim.x;
+        if (wn < nwork) {
+          const int2 it = work[wn];
+          rl.issue(runs[it.x], it.y * NV, lt);   // the next job's, in flight during this one
+        }
+        __syncthreads();   // staging ready
+#pragma unroll 1
+        for (int b = 0; b < kRowBodyBarriers; ++b) __syncthreads();
+        __syncthreads();   // job done
+      }
+    } else {
+      for (int w = L; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];          // (run, job)
+        __syncthreads();   // staging ready
+        row_body<NFIX>(runs[it.x], it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1]);
+        __syncthreads();   // job done: wave buffers free for the next staging
+      }
+    }
+  } else {
+    for (int w = L; w < nwork; w += gridDim.x) {
+      const int2 it = work[w];          // (run, job)
+      row_dispatch(runs[it.x], it.y, sm, twq);
+      __syncthreads();
+    }
   }
 }
 
@@ -459,7 +509,7 @@ __global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, c
 
 template <int NFIX>
 int launch_row_n(const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw, hipStream_t st) {
-  hipLaunchKernelGGL(k_row<NFIX>, dim3(grid), dim3(384), 0, st, runs, work, nwork, tw);
+  hipLaunchKernelGGL(k_row<NFIX>, dim3(grid), dim3(row_threads<NFIX>()), 0, st, runs, work, nwork, tw);
   QS_CHECK_LAUNCH();
   return QS_OK;
 }

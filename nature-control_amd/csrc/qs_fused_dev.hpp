@@ -141,16 +141,62 @@ __device__ __forceinline__ void stage_rows(double2* const* arr, const RunDesc& a
   __syncthreads();
 }
 
+// Row loader of the fixed-size row kernels: two extra waves (threads
+// 384..511, loader thread lt = tid - 384) hold the next job's inverse-set rows
+// in registers while the six compute waves run the current job, and write
+// them to the staging area (the layout of stage_rows) once that job has
+// released the wave buffers.  Only these waves wait on the loads; a compute
+// wave's wait would also drain its own pending stores (loads and stores share
+// one counter).  S = ceil(RH / 128) slots per array and thread.
+template <int N>
+struct RowLoader {
+  static constexpr int NV = WaveLay<N>::NV, H = N / 2 + 1, RH = NV * H, S = (RH + 127) / 128;
+  double2 ld[7][S];
+  double w00, alu;   // the run's W(0,0) and the step's uniform activity
+  // rows row0 .. row0+NV-1 of a's inverse set (out-of-range slots read a
+  // clamped, valid address and are not committed)
+  __device__ __forceinline__ void issue(const RunDesc& a, int row0, int lt) {
+    w00 = gload(a.w00);
+    alu = a.alpha_p ? gload(a.alpha_p) : a.alpha;
+    unsigned off[S];
+#pragma unroll
+    for (int h = 0; h < S; ++h) {
+      const int r = lt + 128 * h;
+      const int rr = r < RH ? r : RH - 1;
+      const int vec = rr / H, k = rr % H;
+      const int y = row0 + vec;
+      off[h] = (unsigned)((y < a.Ny ? y : a.Ny - 1) * a.pitch + k);
+    }
+#pragma unroll
+    for (int t = 0; t < 7; ++t)
+#pragma unroll
+      for (int h = 0; h < S; ++h) ld[t][h] = gload(a.minp[t] + off[h]);
+  }
+  // rows into the staging area, (w00, alu) into sc[0..1]
+  __device__ __forceinline__ void commit(double2* sm, double* sc, int lt) const {
+    if (lt == 0) {
+      sc[0] = w00;
+      sc[1] = alu;
+    }
+#pragma unroll
+    for (int t = 0; t < 7; ++t)
+#pragma unroll
+      for (int h = 0; h < S; ++h)
+        if (lt + 128 * h < RH) sm[t * RH + lt + 128 * h] = ld[t][h];
+  }
+};
+
 // Physical fields of a row job from the staged inverse set (after stage_rows
 // of the inverse-set arrays).  Waves 0-4 build two-for-one C2R inputs, pairing
 // fields of like magnitude, and run the inverse x FFTs; the fields, scaled by
 // 1/(Nx Ny), land in wave buffers 0-4 as pairs
 //   0: (w, dy u + dx v)   1: (u, v)   2: (r1, r2)   3: (L1, L2)   4: (dx u, 0)
 // (vector g of wave buffer q at sm[q*kBuf + g*VS + swz(x)]).  v: scratch
-// registers of the caller.  Ends with a barrier.
+// registers of the caller; w00: the run's W(0,0) (*a.w00).  Ends with a
+// barrier.
 template <int N, class TW>
 __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, double2* sm, const TW& tw,
-                                               double2 (&v)[16]) {
+                                               double2 (&v)[16], double w00) {
   using L = WaveLay<N>;
   constexpr int NV = L::NV, LPC = L::LPC, VS = L::VS, E = L::E, H = N / 2 + 1, RH = NV * H;
   const int tid = threadIdx.x;
@@ -159,7 +205,6 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
   double2* vb = sm + wave * kBuf + g * VS;
   auto arr = [&](int t) -> const double2* { return sm + t * RH + g * H; };   // staged array t, vector g
   if (wave < 5) {
-    const double w00 = gload(a.w00);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int k = lp + LPC * m;
