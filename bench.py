@@ -300,9 +300,20 @@ def cpu_baseline(N, dt, nsteps, q11, q12):
     t0 = time.perf_counter()
     s.step(nsteps, O.IFRK2)
     el = time.perf_counter() - t0
-    return {"value": nsteps / el, "unit": "steps/s", "cores": O.get_threads(), "kind": "port",
+    threads = O.get_threads()
+    # the same port on one core, over a tenth of the sample
+    n1 = max(1, nsteps // 10)
+    O.set_threads(1)
+    s.set_state(np.zeros(m.spec_shape, complex), m.fwd(q11), m.fwd(q12))
+    t0 = time.perf_counter()
+    s.step(n1, O.IFRK2)
+    el1 = time.perf_counter() - t0
+    O.set_threads(cores)
+    return {"value": nsteps / el, "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": f"{nsteps} IFRK2 steps at {N}^2 FP64 (same initial state), "
-                      f"{el:.2f} s wall on {O.get_threads()} OpenMP threads"}
+                      f"{el:.2f} s wall on {threads} OpenMP threads",
+            "single_core": {"value": n1 / el1, "unit": "steps/s", "cores": 1,
+                            "sample": f"{n1} IFRK2 steps from the same initial state, {el1:.2f} s wall"}}
 
 
 def driver_grid_bench(fused_steps=100, unfused_steps=10):
