@@ -74,17 +74,25 @@ def copy_calibrate(mib):
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}), flush=True)
 
 
+# Sources of the fused kernels, in the order their sha256 is taken.
+FUSED_SOURCES = ("qs_fused.hip", "qs_fused_dev.hpp", "qs_fused.hpp", "qs_internal.hpp")
+
+
 def traffic_from_profile(kernel, N):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (profiles/fused_kernels_<N>_traffic.md: rocprofv3 FETCH_SIZE and WRITE_SIZE
     passes scaled by a plain-copy calibration), used only when it was measured
-    on the current kernel source (its sha256 is recorded in the summary)."""
+    on the current kernel sources (the sha256 of the four fused-kernel files,
+    concatenated in FUSED_SOURCES order, is recorded in the summary)."""
     import hashlib
     path = os.path.join(ROOT, "profiles", f"fused_kernels_{N}_traffic.md")
-    src = os.path.join(ROOT, "nature-control_amd", "csrc", "qs_fused.hip")
-    if not (os.path.exists(path) and os.path.exists(src)):
+    srcs = [os.path.join(ROOT, "nature-control_amd", "csrc", f) for f in FUSED_SOURCES]
+    if not (os.path.exists(path) and all(os.path.exists(f) for f in srcs)):
         return None
-    sha = hashlib.sha256(open(src, "rb").read()).hexdigest()
+    h = hashlib.sha256()
+    for f in srcs:
+        h.update(open(f, "rb").read())
+    sha = h.hexdigest()
     vals = {}
     for line in open(path):
         if line.startswith("- ") and ": " in line:
@@ -457,6 +465,14 @@ def main():
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profile(dom, N)}
     step_bytes = 68 * H
+    # Counter traffic per step (rocprofv3 FETCH_SIZE + WRITE_SIZE of each class,
+    # from the committed profile of this kernel source): two row passes, the two
+    # column stages, and the restart prologue once per restart_every steps.
+    # Over the step time it is the time-weighted counter bandwidth of the kernels.
+    per_step = {"row_rhs": 2.0, "col_stage1": 1.0, "col_stage2": 1.0,
+                "col_prologue": 1.0 / args.restart_every if args.restart_every > 0 else 0.0}
+    tr = {k: traffic_from_profile(k, N) for k in per_step}
+    counter_bytes = None if any(v is None for v in tr.values()) else sum(per_step[k] * tr[k] for k in per_step)
     adj = adjoint_bench() if args.adjoint else None
     ens = ensemble_bench() if args.ensemble else None
     badj = batched_adjoint_bench() if args.batched_adjoint else None
@@ -486,7 +502,11 @@ def main():
         "kernels": kernels,
         "step_hbm": {"alg_bytes_per_step": step_bytes, "GBps": round(step_bytes * steps_per_s / 1e9, 1),
                      "frac_of_peak": round(step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
-                     "canonical_101H_frac": round(101 * H * steps_per_s / 1e9 / HBM_PEAK_GBS, 4)},
+                     "canonical_101H_frac": round(101 * H * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "counter_bytes_per_step": None if counter_bytes is None else round(counter_bytes),
+                     "counter_GBps": None if counter_bytes is None else round(counter_bytes * steps_per_s / 1e9, 1),
+                     "counter_frac_of_peak": None if counter_bytes is None
+                     else round(counter_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS, 4)},
         "finite": finite,
         "ms_per_step_with_kernel_events": None if ms_prof is None else round(ms_prof / args.steps, 5),
         "adjoint": adj,
