@@ -8,7 +8,9 @@
 // first pass in registers, then LDS Stockham passes ordered by wave-level
 // synchronisation only):
 //
-//  row job  (6 waves, one job = NV rows: 1024/N for N = 2^a, 768/N for 3 * 2^a):
+//  row job  (6 compute waves, one job = NV rows: 1024/N for N = 2^a, 768/N
+//    for 3 * 2^a; fixed-size kernels add 2 loader waves that fetch the next
+//    job's rows):
 //    7 mixed arrays (x-spectral, y-physical) -> five two-for-one x C2R
 //    (waves 0-4) -> nine physical fields in LDS -> pointwise products in place
 //    -> six two-for-one x R2C (waves 0-5) -> 7 mixed arrays.
@@ -36,9 +38,10 @@
 namespace qs {
 namespace {
 
-// Row job layout (one job = NV = WaveLay<N>::NV rows, 6 waves, 6 LDS buffers):
+// Row job layout (one job = NV = WaveLay<N>::NV rows, 6 compute waves, 6 LDS
+// buffers):
 //  1. the 7 mixed rows are loaded cooperatively (coalesced) into a staging
-//     view of the buffers;
+//     view of the buffers (stage_rows, or the loader waves' RowLoader);
 //  2. waves 0-4 build their two-for-one C2R inputs from the staging area,
 //     pairing fields of like magnitude: (w, dy u + dx v), (u, v), (r1, r2),
 //     (L1, L2), (dx u, -);
