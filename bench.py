@@ -434,6 +434,9 @@ def main():
         return start.elapsed_time(end)
     # 1. the headline: the K steps alone, no per-launch instrumentation
     ms = timed_pass(False)
+    # the same pass four more times (fused path): the median of five K-step runs
+    reps = [ms] + ([] if args.unfused else [timed_pass(False) for _ in range(4)])
+    rep_rates = sorted(args.steps / (t / 1e3) for t in reps)
     # 2. the per-kernel breakdown: the same K steps again with a HIP event pair
     #    around every launch on the solver's stream (each pair adds the
     #    dispatch gap, a few us, to the launch it brackets)
@@ -507,6 +510,8 @@ def main():
                      "counter_GBps": None if counter_bytes is None else round(counter_bytes * steps_per_s / 1e9, 1),
                      "counter_frac_of_peak": None if counter_bytes is None
                      else round(counter_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS, 4)},
+        "repeats": {"steps_per_s": [round(r, 1) for r in rep_rates], "median": round(rep_rates[len(rep_rates) // 2], 2),
+                    "note": "five uninstrumented K-step passes, the first of which is value"},
         "finite": finite,
         "ms_per_step_with_kernel_events": None if ms_prof is None else round(ms_prof / args.steps, 5),
         "adjoint": adj,
