@@ -146,11 +146,6 @@ __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm
   }
 }
 
-template <int N, class TW>
-__device__ __forceinline__ void row_job(const RunDesc& a, int row0, double2* sm, const TW& tw) {
-  stage_rows<N>(a.minp, a, row0, sm);
-  row_body<N>(a, row0, sm, tw, gload(a.w00), a.alpha_p ? gload(a.alpha_p) : a.alpha);
-}
 
 // ===========================================================================
 // Column job
@@ -379,53 +374,16 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
 // Kernels: one run (grid over jobs) and an ensemble (work list over runs).
 // ===========================================================================
 
-template <int N, class TW>
-__device__ __forceinline__ void row_dispatch_n(const RunDesc& a, int job, double2* sm, const TW& tw) {
-  row_job<N>(a, job * WaveLay<N>::NV, sm, tw);
-}
-
-__device__ void row_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq) {
-  switch (a.Nx) {
-    case 16: row_dispatch_n<16>(a, job, sm, TwN<16>{twq}); break;
-    case 32: row_dispatch_n<32>(a, job, sm, TwN<32>{twq}); break;
-    case 64: row_dispatch_n<64>(a, job, sm, TwN<64>{twq}); break;
-    case 128: row_dispatch_n<128>(a, job, sm, TwN<128>{twq}); break;
-    case 256: row_dispatch_n<256>(a, job, sm, TwN<256>{twq}); break;
-    case 512: row_dispatch_n<512>(a, job, sm, TwN<512>{twq}); break;
-    default: row_dispatch_n<1024>(a, job, sm, TwN<1024>{twq}); break;
-  }
-}
-
-template <int MODE>
-__device__ void col_dispatch(const RunDesc& a, int job, double2* sm, const double2* twq, double2* dbuf, bool pro) {
-  const int tc = 4 * wave_nv(a.Ny);
-  const int ntiles = (a.Nxh + tc - 1) / tc;
-  int fld, tile;
-  col_job_map(job, ntiles, fld, tile);
-  const int i0 = tile * tc;
-  switch (a.Ny) {
-    case 16: col_job<16, MODE>(a, fld, i0, sm, TwN<16>{twq}, dbuf, pro); break;
-    case 32: col_job<32, MODE>(a, fld, i0, sm, TwN<32>{twq}, dbuf, pro); break;
-    case 64: col_job<64, MODE>(a, fld, i0, sm, TwN<64>{twq}, dbuf, pro); break;
-    case 128: col_job<128, MODE>(a, fld, i0, sm, TwN<128>{twq}, dbuf, pro); break;
-    case 256: col_job<256, MODE>(a, fld, i0, sm, TwN<256>{twq}, dbuf, pro); break;
-    case 512: col_job<512, MODE>(a, fld, i0, sm, TwN<512>{twq}, dbuf, pro); break;
-    default: col_job<1024, MODE>(a, fld, i0, sm, TwN<1024>{twq}, dbuf, pro); break;
-  }
-}
-
-// NFIX > 0: every run of the work list has Nx = Ny = NFIX (one instantiation),
-// six compute waves and two loader waves (RowLoader) per block;
-// NFIX = 0: per-run dispatch on the mesh size (ensembles of uneven runs), six
-// waves that stage their own rows.
-template <int NFIX>
-constexpr int row_threads() { return NFIX > 0 ? 512 : 384; }
+// Every run of a row kernel's work list has Nx = NFIX (its x FFTs; the row
+// count is a runtime value): six compute waves and two loader waves
+// (RowLoader) per block.
+constexpr int kRowThreads = 512;
 
 template <int NFIX>
-__global__ __launch_bounds__(row_threads<NFIX>()) void k_row(const RunDesc* __restrict__ runs,
-                                                             const int2* __restrict__ work, int nwork,
-                                                             const double2* __restrict__ tw1024) {
-  constexpr int NT = row_threads<NFIX>();
+__global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
+                                                    int nwork, const double2* __restrict__ tw1024) {
+  static_assert(NFIX > 0, "one instantiation per x size");
+  constexpr int NT = kRowThreads;
   __shared__ double2 sm[6 * kBuf];
   __shared__ double2 twq[kTwq];
   __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
@@ -437,7 +395,7 @@ __global__ __launch_bounds__(row_threads<NFIX>()) void k_row(const RunDesc* __re
     __syncthreads();
   }
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  if constexpr (NFIX > 0) {
+  {
     constexpr int NV = WaveLay<NFIX>::NV;
     if (threadIdx.x >= 384) {
       // Loader waves.  Per job they pass exactly the compute waves' block
@@ -468,12 +426,6 @@ __global__ __launch_bounds__(row_threads<NFIX>()) void k_row(const RunDesc* __re
         __syncthreads();   // job done: wave buffers free for the next staging
       }
     }
-  } else {
-    for (int w = L; w < nwork; w += gridDim.x) {
-      const int2 it = work[w];          // (run, job)
-      row_dispatch(runs[it.x], it.y, sm, twq);
-      __syncthreads();
-    }
   }
 }
 
@@ -487,7 +439,9 @@ __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double
   col_job<NY, MODE>(a, fld, tile * tc, sm, tw, dbuf, pro);
 }
 
-// pro != 0: the prologue of the final-stage modes (S2, R4, E).
+// Every run of a column kernel's work list has Ny = NFIX (its y FFTs; the
+// column count is a runtime value).  pro != 0: the prologue of the
+// final-stage modes (S2, R4, E).
 template <int NFIX, int MODE>
 __global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
                                              int nwork, const double2* __restrict__ tw1024, int pro) {
@@ -504,15 +458,14 @@ __global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, c
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   for (int w = L; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
-    if constexpr (NFIX > 0) col_dispatch_n<NFIX, MODE>(runs[it.x], it.y, sm, fused_tw<NFIX>(twq, twt), dbuf, pro != 0);
-    else col_dispatch<MODE>(runs[it.x], it.y, sm, twq, dbuf, pro != 0);
+    col_dispatch_n<NFIX, MODE>(runs[it.x], it.y, sm, fused_tw<NFIX>(twq, twt), dbuf, pro != 0);
     __syncthreads();
   }
 }
 
 template <int NFIX>
 int launch_row_n(const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw, hipStream_t st) {
-  hipLaunchKernelGGL(k_row<NFIX>, dim3(grid), dim3(row_threads<NFIX>()), 0, st, runs, work, nwork, tw);
+  hipLaunchKernelGGL(k_row<NFIX>, dim3(grid), dim3(kRowThreads), 0, st, runs, work, nwork, tw);
   QS_CHECK_LAUNCH();
   return QS_OK;
 }
@@ -562,7 +515,8 @@ int col_jobs(const RunDesc& a) {
     case 256: return FN<256>(__VA_ARGS__);          \
     case 512: return FN<512>(__VA_ARGS__);          \
     case 1024: return FN<1024>(__VA_ARGS__);        \
-    default: return FN<0>(__VA_ARGS__);             \
+    default: return fail(QS_ERR_UNSUPPORTED_SIZE,   \
+                         "fused kernels: side %d", n); \
   }
 
 int launch_fused_row(int nfix, const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw1024,

@@ -51,8 +51,9 @@ int row_jobs(const RunDesc& a);
 int col_jobs(const RunDesc& a);
 
 // work[k] = (run index, job index).  grid <= nwork; blocks loop over the list.
-// nfix: the common mesh size of all listed runs (Nx = Ny = nfix), or 0 for a
-// list of runs of different sizes (per-run dispatch inside the kernel).
+// nfix: the size common to all listed runs in the kernel's FFT direction (Nx
+// for the row pass, Ny for the column stages): a power of two 16..1024 or
+// 3 * 2^a 48..768; any other value is QS_ERR_UNSUPPORTED_SIZE.
 int launch_fused_row(int nfix, const RunDesc* runs, const int2* work, int nwork, int grid,
                      const double2* tw1024, hipStream_t st);
 int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, int nwork, int grid,

@@ -64,12 +64,12 @@ __device__ __forceinline__ void build_twtab(double2* t, const double2* q, int ti
   }
 }
 
-// The fused kernels of fixed size NFIX use the tables for N = 3 * 2^a, whose
+// The fused kernels of size NFIX use the tables for N = 3 * 2^a, whose
 // radix-4 pass chain is long (768: 4% faster per step); for N = 2^a the
 // computed factors are as fast or faster (1024: the row kernel 2.5 us slower
-// with tables), and NFIX = 0 (per-run sizes) has no single table.
+// with tables).
 template <int NFIX>
-constexpr bool use_twt() { return NFIX > 0 && NFIX % 3 == 0; }
+constexpr bool use_twt() { return NFIX % 3 == 0; }
 // LDS entries such a kernel reserves for its tables (1 when it has none).
 template <int NFIX>
 constexpr int twt_len() {

@@ -52,7 +52,8 @@ int upload_desc(qs_solver* S) {
   return QS_OK;
 }
 
-int nfix_of(const Mesh* m) { return m->Nx == m->Ny ? m->Nx : 0; }
+// The row pass's kernels are instantiated by Nx (their x FFTs), the column
+// stages' by Ny: a non-square mesh runs the same fixed-size kernels.
 // Row pass blocks: one per compute unit (the kernel's LDS admits one block
 // per CU), each looping over its jobs.
 int row_grid(const qs_solver* S) {
@@ -75,12 +76,12 @@ int fused_check(const qs_solver* S) {
 const double2* fused_tw(const Mesh* m) { return m->Nx % 3 == 0 ? m->priv->tw768 : m->priv->tw1024; }
 int row_rhs_raw(qs_solver* S) {
   QS_TRY(fused_check(S));
-  return launch_fused_row(nfix_of(S->mesh), S->d_desc, S->d_work, S->nrow, row_grid(S), fused_tw(S->mesh),
+  return launch_fused_row(S->mesh->Nx, S->d_desc, S->d_work, S->nrow, row_grid(S), fused_tw(S->mesh),
                           stream_of(S->mesh));
 }
 int col_stage_raw(qs_solver* S, int mode) {
   QS_TRY(fused_check(S));
-  return launch_fused_col(nfix_of(S->mesh), mode, S->d_desc, S->d_work, S->ncol, S->ncol, fused_tw(S->mesh),
+  return launch_fused_col(S->mesh->Ny, mode, S->d_desc, S->d_work, S->ncol, S->ncol, fused_tw(S->mesh),
                           stream_of(S->mesh));
 }
 
