@@ -33,8 +33,8 @@ def bandlimited(m, rng, kmax, rms):
     return f * rms / np.sqrt(np.mean(f * f))
 
 
-def setup(N=32, dt=1e-3, seed=3, activity=None, advect_w=1):
-    m = O.Mesh(N, N, 2 * np.pi)
+def setup(N=32, dt=1e-3, seed=3, activity=None, advect_w=1, Ny=None):
+    m = O.Mesh(N, N if Ny is None else Ny, 2 * np.pi)
     s = O.Solver(m, O.params_array(lam=P["lam"], Re=P["Re"], Er=P["Er"], Rf=P["Rf"], alpha=P["alpha"],
                                    DQ=P["DQ"], aQ=P["aQ"], cQ=P["cQ"], dt=dt, advect_w=advect_w))
     rng = np.random.default_rng(seed)
@@ -50,9 +50,9 @@ def rand_spec(m, rng):
     return rng.standard_normal(m.spec_shape) + 1j * rng.standard_normal(m.spec_shape)
 
 
-@pytest.mark.parametrize("N", [8, 12, 32])
-def test_transform_transposes(N):
-    m, s, _, rng = setup(N)
+@pytest.mark.parametrize("N,Ny", [(8, 8), (12, 12), (32, 32), (16, 8), (8, 32)])
+def test_transform_transposes(N, Ny):
+    m, s, _, rng = setup(N, Ny=Ny)
     A = O.Adjoint(s)
     for _ in range(3):
         G = rand_spec(m, rng)            # arbitrary, non-Hermitian edge columns included
@@ -96,8 +96,9 @@ def test_rhs_vjp_finite_difference(advect_w):
     assert abs(fd - an) <= 1e-8 * abs(an), (fd, an)
 
 
-def test_step_vjp_finite_difference():
-    m, s, X, rng = setup(32, dt=2e-3)
+@pytest.mark.parametrize("N,Ny", [(32, 32), (32, 16), (16, 64)])
+def test_step_vjp_finite_difference(N, Ny):
+    m, s, X, rng = setup(N, dt=2e-3, Ny=Ny)
     A = O.Adjoint(s)
     gb = [rand_spec(m, rng) for _ in range(3)]
     ub, ab = A.step_vjp(X, gb)
