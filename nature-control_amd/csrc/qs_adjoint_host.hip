@@ -211,7 +211,7 @@ int launch_row(AdjGroup* G, const RunDesc* d, hipStream_t st) {   // persistent:
 }
 int launch_col(AdjGroup* G, const RunDesc* d, int mode, hipStream_t st) {
   const Mesh* m = mesh_of(G);
-  return launch_fused_col(m->Nx, mode, d, G->d_col, G->ncol, G->ncol, m->priv->tw1024, st);
+  return launch_fused_col(m->Ny, mode, d, G->d_col, G->ncol, G->ncol, m->priv->tw1024, st);
 }
 int launch_arow(AdjGroup* G, const RunDesc* d, hipStream_t st) {
   const Mesh* m = mesh_of(G);
@@ -219,7 +219,7 @@ int launch_arow(AdjGroup* G, const RunDesc* d, hipStream_t st) {
 }
 int launch_acol(AdjGroup* G, const RunDesc* d, int amode, hipStream_t st) {
   const Mesh* m = mesh_of(G);
-  return launch_adj_col(m->Nx, amode, d, G->d_col, G->ncol, G->ncol, m->priv->tw1024, st);
+  return launch_adj_col(m->Ny, amode, d, G->d_col, G->ncol, G->ncol, m->priv->tw1024, st);
 }
 
 double2* slot(const qs_solver* S, int b, int s, int q) {
@@ -426,9 +426,11 @@ int group_backward(AdjGroup* G) {
   return QS_OK;
 }
 
-int square_check(const Mesh* m) {
-  if (m->Nx != m->Ny) return fail(QS_ERR_UNSUPPORTED_SIZE, "adjoint: square meshes only");
-  if (!fused_side(m->Nx)) return fail(QS_ERR_UNSUPPORTED_SIZE, "adjoint: mesh %d (powers of two, 16..1024)", m->Nx);
+// The adjoint's row kernels are instantiated by Nx, its column kernels by Ny:
+// both sides powers of two, 16..1024 (not necessarily equal).
+int size_check(const Mesh* m) {
+  if (!fused_side(m->Nx) || !fused_side(m->Ny))
+    return fail(QS_ERR_UNSUPPORTED_SIZE, "adjoint: mesh %dx%d (sides powers of two, 16..1024)", m->Nx, m->Ny);
   return QS_OK;
 }
 
@@ -444,7 +446,7 @@ AdjGroup* adj_group_new(const std::vector<qs_solver*>& runs) {
 }
 void adj_group_free(AdjGroup* G) { group_free(G); }
 int adj_group_forward(AdjGroup* G, int nsteps, int k, const double* alpha_t) {
-  QS_TRY(square_check(mesh_of(G)));
+  QS_TRY(size_check(mesh_of(G)));
   return group_forward(G, nsteps, k, alpha_t);
 }
 int adj_group_backward(AdjGroup* G) {
@@ -502,7 +504,7 @@ int qs_adj_memory(qs_solver* S, size_t* bytes) {
 int qs_adj_forward(qs_solver* S, int nsteps, int ckpt_every, const Qreal* alpha_t) {
   if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
   if (nsteps <= 0) return fail(QS_ERR_INVALID_ARG, "qs_adj_forward: nsteps must be > 0");
-  QS_TRY(square_check(S->mesh));
+  QS_TRY(size_check(S->mesh));
   const int k = ckpt_every <= 0 ? nsteps : std::min(ckpt_every, nsteps);
   if (!S->adj) S->adj = new AdjState();   // sized by the group's first forward
   if (!S->adj->solo) S->adj->solo = adj_group_new({S});
