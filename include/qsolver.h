@@ -25,7 +25,8 @@
  *  - Supported meshes: Nx, Ny = 2^a in [16, 2048] or 3 * 2^a in [48, 1536]
  *    (the per-op calls and the unfused steps); the fused steps take powers of
  *    two up to 1024 and square 3 * 2^a meshes up to 768 (the reference
- *    driver's grid); the adjoint and ensembles take powers of two up to 1024.
+ *    driver's grid); the adjoint takes power-of-two sides up to 1024 (Nx and
+ *    Ny independently), ensembles square power-of-two runs up to 1024.
  */
 #ifndef QSOLVER_H
 #define QSOLVER_H
