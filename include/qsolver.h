@@ -230,7 +230,8 @@ int qs_solver_debug_phase(qs_solver* s, int phase);
 int qs_solver_debug_array(qs_solver* s, int which, Qcomp* out);
 /* Test hook: the fused kernels' per-wavefront FFT (length n = 2^a in
  * [16, 1024] or 3 * 2^a in [48, 768], sign -1 forward / +1 inverse,
- * unnormalised) on the 1024/n (resp. 768/n) contiguous vectors of one wave. */
+ * unnormalised) on 1024/n (resp. 768/n) contiguous vectors, transformed one
+ * wave's worth of vectors at a time. */
 int qs_debug_wave_fft(const Mesh* mesh, int n, int sign, const Qcomp* in, Qcomp* out);
 
 /* ---- discrete adjoint of the IFRK2 step ----------------------------------

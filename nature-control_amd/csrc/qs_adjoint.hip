@@ -38,7 +38,7 @@ __device__ __forceinline__ double wave_sum(double s) {
 }
 
 // ===========================================================================
-// Row adjoint job (one job = NV = 1024/N rows, 6 waves):
+// Row adjoint job (one job = NV = WaveLay<N>::NV rows, 6 waves):
 //  A. Fbar rows -> transposed x combinations -> (x R2C)^T: physical adjoints
 //     of the 11 products, as 6 pairs (waves 0-5), kept in the block's scratch
 //     slot (global memory: they do not fit LDS beside the fields);
@@ -51,7 +51,8 @@ __device__ __forceinline__ double wave_sum(double s) {
 template <int N>
 __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm, const double2* twq,
                                          double2* scr) {
-  constexpr int NV = 1024 / N, LPC = N / 16, VS = N + N / 16, H = N / 2 + 1, RH = NV * H;
+  using L = WaveLay<N>;
+  constexpr int NV = L::NV, LPC = L::LPC, VS = L::VS, E = L::E, H = N / 2 + 1, RH = NV * H;
   const TwN<N> tw{twq};
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -65,7 +66,7 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
   {
     const double2* st = sm + g * H;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
+    for (int m = 0; m < E; ++m) {
       const int k = lp + LPC * m;
       const int bin = k > N / 2 ? N - k : k;
       const double kx = a.dkx * bin;
@@ -104,7 +105,7 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
   __syncthreads();   // staging area free
   wave_fft<N, +1>(v, vb, lp, tw);
 #pragma unroll
-  for (int m = 0; m < 16; ++m) gstore(scr + wave * 1024 + g * N + lp + LPC * m, v[m]);
+  for (int m = 0; m < E; ++m) gstore(scr + wave * 1024 + g * N + lp + LPC * m, v[m]);
 
   // ---- B. the stage's physical fields (buffers 0-4) -------------------------
   stage_rows<N>(a.minp, a, row0, sm);   // its barrier also publishes the scratch writes
@@ -113,7 +114,7 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
   // ---- C. pointwise Jacobian transposed (friction/Qsolver.cuh:63-300) -------
   const double alu = a.alpha_p ? gload(a.alpha_p) : a.alpha;
   const double lam = a.lambda, DQ = a.DQ, aQ = a.aQ, cQ = a.cQ;
-  for (int p = tid; p < 1024; p += 384) {
+  for (int p = tid; p < NV * N; p += 384) {
     const int gg = p / N, x = p % N;
     const int o = gg * VS + swz(x);
     const int yy = row0 + gg;
@@ -193,11 +194,11 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
   // ---- D. (x C2R)^T of the field adjoints -> Mbar ---------------------------
   if (wave < 5) {
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = vb[swz(lp + LPC * m)];
+    for (int m = 0; m < E; ++m) v[m] = vb[swz(lp + LPC * m)];
     wave_sync();
     wave_fft<N, -1>(v, vb, lp, tw);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) vb[swz(lp + LPC * m)] = v[m];
+    for (int m = 0; m < E; ++m) vb[swz(lp + LPC * m)] = v[m];
   }
   __syncthreads();
   const double sc = a.scale;
@@ -236,7 +237,7 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
 template <int NY, int AMODE>
 __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq,
                                          double2* dbuf) {
-  constexpr int LPC = NY / 16;
+  constexpr int LPC = WaveLay<NY>::LPC, E = WaveLay<NY>::E;
   const TwN<NY> tw{twq};
   ColTile<NY> T(a, sm, i0);
   const int lp = T.lp;
@@ -274,7 +275,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
   if (AMODE == AMODE_START) {
     T.load(a.lam_[FLD]);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
+    for (int m = 0; m < E; ++m) {
       const int j = lp + LPC * m;
       double2* sl = vb + swz(j);
       const double c = keep(j) ? 1.0 : 0.0;
@@ -305,7 +306,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
         D.issue(a.mix[2]);
         wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {   // psi = W / k^2
+        for (int m = 0; m < E; ++m) {   // psi = W / k^2
           const int j = lp + LPC * m;
           const double ky = kyj(j);
           const double k2 = kx * kx + ky * ky;
@@ -321,7 +322,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
         D.issue(state_in);
         wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {   // i ky psi -> conj: -i ky / k^2
+        for (int m = 0; m < E; ++m) {   // i ky psi -> conj: -i ky / k^2
           const int j = lp + LPC * m;
           const double ky = kyj(j);
           const double k2 = kx * kx + ky * ky;
@@ -345,7 +346,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       D.issue(state_in);
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < E; ++m) {
         const double ky = kyj(lp + LPC * m);
         st[m] = make_double2(st[m].x - ky * ky * v[m].x, st[m].y - ky * ky * v[m].y);
       }
@@ -356,7 +357,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     D.wait();
     if (AMODE == AMODE_B) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
         const double2 lamv = D.at(j, cl);
@@ -367,7 +368,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       T.store(a.acc_[FLD]);
     } else if (AMODE == AMODE_A) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
         const double c = keep(j) ? 1.0 : 0.0;
@@ -379,7 +380,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       T.store(a.lam_[FLD]);
     } else {   // AMODE_END
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
+      for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         const double2 acc = D.at(j, cl);
         vb[swz(j)] = make_double2(acc.x + st[m].x, acc.y + st[m].y);
@@ -393,21 +394,21 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
   if (FLD == 0) {
     const int nyq_x = a.Nx / 2;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {   // real part at the self-conjugate bins
+    for (int m = 0; m < E; ++m) {   // real part at the self-conjugate bins
       const int j = lp + LPC * m;
       if ((i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
     }
     {
       double2 v[16];
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = st[m];
+      for (int m = 0; m < E; ++m) v[m] = st[m];
       wave_fft<NY, +1>(v, vb, lp, tw);
       T.stage_out(a.mix[0], v);
     }
     {
       double2 v[16];
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {   // -i ky
+      for (int m = 0; m < E; ++m) {   // -i ky
         const double ky = kyj(lp + LPC * m);
         v[m] = make_double2(ky * st[m].y, -ky * st[m].x);
       }
@@ -415,7 +416,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       T.stage_out(a.mix[1], v);
     }
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {     // ky^2
+    for (int m = 0; m < E; ++m) {     // ky^2
       const double ky = kyj(lp + LPC * m);
       st[m] = make_double2(ky * ky * st[m].x, ky * ky * st[m].y);
     }
@@ -426,12 +427,12 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     {
       double2 v[16];
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = st[m];
+      for (int m = 0; m < E; ++m) v[m] = st[m];
       wave_fft<NY, +1>(v, vb, lp, tw);
       T.stage_out(a.mix[m0], v);
     }
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {     // -i ky
+    for (int m = 0; m < E; ++m) {     // -i ky
       const double ky = kyj(lp + LPC * m);
       st[m] = make_double2(ky * st[m].y, -ky * st[m].x);
     }
@@ -454,7 +455,7 @@ __global__ __launch_bounds__(384) void k_arow(const RunDesc* __restrict__ runs, 
   for (int w = L; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
     const RunDesc& a = runs[it.x];
-    arow_job<N>(a, it.y * (1024 / N), sm, twq, a.scratch + (size_t)blockIdx.x * 6 * 1024);
+    arow_job<N>(a, it.y * WaveLay<N>::NV, sm, twq, a.scratch + (size_t)blockIdx.x * 6 * 1024);
     __syncthreads();
   }
 }
@@ -468,7 +469,7 @@ __global__ __launch_bounds__(256) void k_acol(const RunDesc* __restrict__ runs, 
   stage_twiddles(twq, tw1024, threadIdx.x, 256);
   __syncthreads();
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  constexpr int tc = 4 * (1024 / NY);
+  constexpr int tc = 4 * WaveLay<NY>::NV;
   for (int w = L; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
     const RunDesc& a = runs[it.x];

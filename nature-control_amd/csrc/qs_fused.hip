@@ -8,9 +8,8 @@
 // first pass in registers, then LDS Stockham passes ordered by wave-level
 // synchronisation only):
 //
-//  row job  (6 compute waves, one job = NV rows: 1024/N for N = 2^a, 768/N
-//    for 3 * 2^a; fixed-size kernels add 2 loader waves that fetch the next
-//    job's rows):
+//  row job  (6 compute waves, one job = NV = 64 E / N rows, E elements per
+//    lane (wave_e); 2 loader waves fetch the next job's rows):
 //    7 mixed arrays (x-spectral, y-physical) -> five two-for-one x C2R
 //    (waves 0-4) -> nine physical fields in LDS -> pointwise products in place
 //    -> six two-for-one x R2C (waves 0-5) -> 7 mixed arrays.
@@ -545,11 +544,14 @@ __global__ __launch_bounds__(64) void k_wave_fft_test(const double2* in, double2
   __syncthreads();
   using L = WaveLay<N>;
   constexpr int CW = L::NV, LPC = L::LPC, VS = L::VS, E = L::E;
+  constexpr int NVEC = (N % 3 == 0 ? 768 : 1024) / N;   // vectors per call, CW per pass
   const int lane = threadIdx.x, lp = lane / CW, g = lane % CW;
-  double2 v[16];
-  for (int m = 0; m < E; ++m) v[m] = in[g * N + lp + LPC * m];
-  wave_fft<N, SIGN>(v, sm + g * VS, lp, TwN<N>{twq});
-  for (int m = 0; m < E; ++m) out[g * N + lp + LPC * m] = v[m];
+  for (int g0 = 0; g0 < NVEC; g0 += CW) {
+    double2 v[16];
+    for (int m = 0; m < E; ++m) v[m] = in[(g0 + g) * N + lp + LPC * m];
+    wave_fft<N, SIGN>(v, sm + g * VS, lp, TwN<N>{twq});
+    for (int m = 0; m < E; ++m) out[(g0 + g) * N + lp + LPC * m] = v[m];
+  }
 }
 template <int N>
 int launch_wave_fft_test(int sign, const double2* in, double2* out, const double2* tw, hipStream_t st) {
