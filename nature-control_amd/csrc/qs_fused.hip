@@ -394,36 +394,34 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
     __syncthreads();
   }
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  {
-    constexpr int NV = WaveLay<NFIX>::NV;
-    if (threadIdx.x >= 384) {
-      // Loader waves.  Per job they pass exactly the compute waves' block
-      // barriers: staging ready, the kRowBodyBarriers of row_body, job done.
-      const int lt = threadIdx.x - 384;
-      RowLoader<NFIX> rl;
-      if (L < nwork) {
-        const int2 it = work[L];
-        rl.issue(runs[it.x], it.y * NV, lt);
+  constexpr int NV = WaveLay<NFIX>::NV;
+  if (threadIdx.x >= 384) {
+    // Loader waves.  Per job they pass exactly the compute waves' block
+    // barriers: staging ready, the kRowBodyBarriers of row_body, job done.
+    const int lt = threadIdx.x - 384;
+    RowLoader<NFIX> rl;
+    if (L < nwork) {
+      const int2 it = work[L];
+      rl.issue(runs[it.x], it.y * NV, lt);
+    }
+    for (int w = L; w < nwork; w += gridDim.x) {
+      rl.commit(sm, rsc, lt);            // this job's rows (waits for the loads)
+      const int wn = w + gridDim.x;
+      if (wn < nwork) {
+        const int2 it = work[wn];
+        rl.issue(runs[it.x], it.y * NV, lt);   // the next job's, in flight during this one
       }
-      for (int w = L; w < nwork; w += gridDim.x) {
-        rl.commit(sm, rsc, lt);            // this job's rows (waits for the loads)
-        const int wn = w + gridDim.x;
-        if (wn < nwork) {
-          const int2 it = work[wn];
-          rl.issue(runs[it.x], it.y * NV, lt);   // the next job's, in flight during this one
-        }
-        __syncthreads();   // staging ready
+      __syncthreads();   // staging ready
 #pragma unroll 1
-        for (int b = 0; b < kRowBodyBarriers; ++b) __syncthreads();
-        __syncthreads();   // job done
-      }
-    } else {
-      for (int w = L; w < nwork; w += gridDim.x) {
-        const int2 it = work[w];          // (run, job)
-        __syncthreads();   // staging ready
-        row_body<NFIX>(runs[it.x], it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1]);
-        __syncthreads();   // job done: wave buffers free for the next staging
-      }
+      for (int b = 0; b < kRowBodyBarriers; ++b) __syncthreads();
+      __syncthreads();   // job done
+    }
+  } else {
+    for (int w = L; w < nwork; w += gridDim.x) {
+      const int2 it = work[w];          // (run, job)
+      __syncthreads();   // staging ready
+      row_body<NFIX>(runs[it.x], it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1]);
+      __syncthreads();   // job done: wave buffers free for the next staging
     }
   }
 }
