@@ -10,7 +10,7 @@ import qsolver as Q  # noqa: E402
 from qsolver import synth  # noqa: E402
 
 BASE = dict(lam=1.0, Re=1.0, Er=1.0, D_Q=0.05, a_Q=1.0, c_Q=1.0, dt=1e-3)
-SIZES = [32, 64, 32, 16, 64, 128]
+SIZES = [32, 64, 32, 16, 64, 128, 256, 512]
 
 
 def run_params(n, seed=3):
