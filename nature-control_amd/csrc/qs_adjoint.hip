@@ -361,9 +361,9 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
         const double2 lamv = D.at(j, cl);
-        const double E = IFof(j);
-        *sl = make_double2(E * (lamv.x + st[m].x), E * (lamv.y + st[m].y));
-        st[m] = make_double2(E * (0.5 * dt * lamv.x + dt * st[m].x), E * (0.5 * dt * lamv.y + dt * st[m].y));
+        const double IFj = IFof(j);
+        *sl = make_double2(IFj * (lamv.x + st[m].x), IFj * (lamv.y + st[m].y));
+        st[m] = make_double2(IFj * (0.5 * dt * lamv.x + dt * st[m].x), IFj * (0.5 * dt * lamv.y + dt * st[m].y));
       }
       T.store(a.acc_[FLD]);
     } else if (AMODE == AMODE_A) {

@@ -4,9 +4,9 @@
 // TimeIntegration/IFRK2.cuh:16-51 (stages, IFh := exp(L dt)),
 // InertialFriction/friction/friction.cu:212-260 (call order).
 //
-// Two kernels per right-hand side, both built on per-wavefront FFTs (radix-16
-// first pass in registers, then LDS Stockham passes ordered by wave-level
-// synchronisation only):
+// Two kernels per right-hand side, both built on per-wavefront FFTs (radix-E
+// first pass in registers, E = wave_e(N) elements per lane, then LDS Stockham
+// passes ordered by wave-level synchronisation only):
 //
 //  row job  (6 compute waves, one job = NV = 64 E / N rows, E elements per
 //    lane (wave_e); 2 loader waves fetch the next job's rows):
@@ -529,8 +529,9 @@ int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, 
 }  // namespace qs
 
 // ---------------------------------------------------------------------------
-// Test hook: per-wavefront FFT of the CW = WaveLay<N>::NV vectors of one wave.
-// in/out: CW vectors of length N, contiguous.
+// Test hook: per-wavefront FFT of NVEC = 1024/N (768/N for N = 3 * 2^a)
+// contiguous vectors of length N, transformed CW = WaveLay<N>::NV vectors (one
+// wave's worth) at a time.
 // ---------------------------------------------------------------------------
 namespace qs {
 namespace {

@@ -270,7 +270,7 @@ struct ColTile {
   }
   __device__ __forceinline__ int slot(int j, int c) const { return (c / CW) * WS + (c % CW) * VS + swz(j); }
   // cooperative coalesced tile load into the wave buffers (ends with a barrier);
-  // all 16 loads are in flight before the first LDS write (the scheduling
+  // all E loads are in flight before the first LDS write (the scheduling
   // barrier keeps the compiler from pairing each load with its write)
   __device__ __forceinline__ void load(const double2* __restrict__ src) {
     double2 ld[E];

@@ -388,14 +388,17 @@ __device__ __forceinline__ void dft12(double2* v) {
 }
 
 // ---------------------------------------------------------------------------
-// Lane layout of the per-wavefront FFT of length-N vectors: E elements per
-// lane, LPC = N/E lanes per vector, NV = 64/LPC vectors per wave (one wave
-// holds NV whole vectors), vector stride VS in a wave buffer (swizzled).
-//   N = 2^a (16..1024):      E = 16, radix-16 first pass, radix-16 passes;
-//   N = 3 * 2^a (48..768):   E = 12, radix-12 first pass, radix-4/2 passes.
+// Lane layout of the per-wavefront FFT of length-N vectors: E = wave_e(N)
+// elements per lane, LPC = N/E lanes per vector, NV = 64/LPC vectors per wave
+// (one wave holds NV whole vectors), vector stride VS in a wave buffer
+// (swizzled).  The first pass is a radix-E DFT in registers, then Stockham
+// passes through LDS of radix RMAX (16 for E = 16, E for E < 16, 4 for
+// 3 * 2^a) while it divides the remaining length:
+//   N = 2^a, N != 256:  E = 16 (radix-16 passes);
+//   N = 256:            E = 8 (radix-8 first pass, then radix-8 and radix-4);
+//   N = 3 * 2^a:        E = 12 (radix-12 = 3 x 4 first pass, radix-4/2 passes).
 // ---------------------------------------------------------------------------
-// Elements per lane of the per-wave FFT of length N: 16 (12 for 3 * 2^a), and
-// 8 at N = 256, where a single run's 256 rows then make 128 row jobs (64 at
+// Why 8 at 256: a single 256^2 run's 256 rows then make 128 row jobs (64 at
 // 16 per lane) and its columns 51 column jobs (27): 1.6x the steps/s of one
 // 256^2 run and 1.55x its adjoint, for 8 % less aggregate rate in ensembles
 // that already fill the device (4 per lane: 2.25x one run, but ensembles

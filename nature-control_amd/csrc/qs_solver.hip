@@ -46,7 +46,9 @@ namespace {
 
 int upload_desc(qs_solver* S) {
   const RunDesc d = make_desc(S);
+  // d is a local: the copy completes before it goes out of scope
   QS_HIP(hipMemcpyAsync(S->d_desc, &d, sizeof(RunDesc), hipMemcpyHostToDevice, stream_of(S->mesh)));
+  QS_HIP(hipStreamSynchronize(stream_of(S->mesh)));
   S->nrow = row_jobs(d);
   S->ncol = col_jobs(d);
   return QS_OK;
@@ -83,6 +85,23 @@ int col_stage_raw(qs_solver* S, int mode) {
   QS_TRY(fused_check(S));
   return launch_fused_col(S->mesh->Ny, mode, S->d_desc, S->d_work, S->ncol, S->ncol, fused_tw(S->mesh),
                           stream_of(S->mesh));
+}
+
+// Non-finite state (friction.cu:280 checks the S field of each snapshot and
+// exits): the final column stages raise the device flag; each fused step call
+// ends with an asynchronous copy of it into pinned host memory, and a later
+// call that finds the copy set refuses to run with QS_ERR_NONFINITE.  Nothing
+// synchronises on the way: the refusal comes at the first call after the copy
+// has landed.  qs_solver_nonfinite synchronises, reports and clears the flag.
+int nonfinite_gate(const qs_solver* S) {
+  if (*(volatile const int*)S->hflag)
+    return fail(QS_ERR_NONFINITE, "non-finite value in the state of an earlier step (qs_solver_nonfinite "
+                "reports and clears the flag)");
+  return QS_OK;
+}
+int nonfinite_publish(qs_solver* S) {
+  QS_HIP(hipMemcpyAsync(S->hflag, S->flag, sizeof(int), hipMemcpyDeviceToHost, stream_of(S->mesh)));
+  return QS_OK;
 }
 
 hipEvent_t take_event(qs_solver* S) {
@@ -194,6 +213,8 @@ int qs_solver_create(Mesh* mesh, const qs_params* params, qs_solver** out) {
   for (int k = 0; k < 7 && ok; ++k) ok = hipMalloc((void**)&S->mix[k], hs) == hipSuccess;
   ok = ok && hipMalloc((void**)&S->w00, sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void**)&S->flag, sizeof(int)) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&S->hflag, sizeof(int), hipHostMallocDefault) == hipSuccess;
+  if (S->hflag) *S->hflag = 0;
   ok = ok && hipMalloc((void**)&S->d_desc, sizeof(RunDesc)) == hipSuccess;
   ok = ok && hipMalloc((void**)&S->d_work, sizeof(int2) * nwork) == hipSuccess;
   if (!ok) {
@@ -238,6 +259,7 @@ int qs_solver_destroy(qs_solver* S) {
     if (S->mix[k]) (void)hipFree(S->mix[k]);
   for (void* p : {(void*)S->Ra, (void*)S->w00, (void*)S->flag, (void*)S->d_desc, (void*)S->d_work})
     if (p) (void)hipFree(p);
+  if (S->hflag) (void)hipHostFree(S->hflag);
   for (auto* f : S->F) qs_field_destroy(f);
   for (auto* t : S->IFt)
     if (t) (void)hipFree(t);
@@ -295,6 +317,7 @@ int qs_solver_get_state(qs_solver* S, Qcomp* w, Qcomp* r1, Qcomp* r2) {
 int qs_step_ifrk2(qs_solver* S, int nsteps) {
   if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
   if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
+  QS_TRY(nonfinite_gate(S));
   for (int n = 0; n < nsteps; ++n) {
     if (!S->mixed_valid) {
       QS_TRY(col_stage(S, MODE_PRO));
@@ -305,7 +328,7 @@ int qs_step_ifrk2(qs_solver* S, int nsteps) {
     QS_TRY(row_rhs(S));
     QS_TRY(col_stage(S, MODE_S2));
   }
-  return QS_OK;
+  return nonfinite_publish(S);
 }
 
 // friction.cu:212-260 verbatim (IF-RK4, RK4.cuh:12-74 with IF = exp(L dt),
@@ -314,6 +337,7 @@ int qs_step_ifrk2(qs_solver* S, int nsteps) {
 int qs_step_rk4(qs_solver* S, int nsteps) {
   if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
   if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
+  QS_TRY(nonfinite_gate(S));
   for (int n = 0; n < nsteps; ++n) {
     if (!S->mixed_valid) {
       QS_TRY(col_stage(S, MODE_PRO_R4));
@@ -324,7 +348,7 @@ int qs_step_rk4(qs_solver* S, int nsteps) {
       QS_TRY(col_stage(S, mode));
     }
   }
-  return QS_OK;
+  return nonfinite_publish(S);
 }
 
 // IF-Euler (TimeIntegration/IFE.cuh:15-25), then dealias: one right-hand side
@@ -332,6 +356,7 @@ int qs_step_rk4(qs_solver* S, int nsteps) {
 int qs_step_ife(qs_solver* S, int nsteps) {
   if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
   if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
+  QS_TRY(nonfinite_gate(S));
   for (int n = 0; n < nsteps; ++n) {
     if (!S->mixed_valid) {
       QS_TRY(col_stage(S, MODE_PRO_E));
@@ -340,7 +365,7 @@ int qs_step_ife(qs_solver* S, int nsteps) {
     QS_TRY(row_rhs(S));
     QS_TRY(col_stage(S, MODE_E));
   }
-  return QS_OK;
+  return nonfinite_publish(S);
 }
 
 // friction.cu:212-260 with the IFRK2 stage kernels and IFh := IF.
@@ -480,9 +505,12 @@ int qs_solver_profile_read(qs_solver* S, double* avg_ms, int* counts) {
 int qs_solver_nonfinite(qs_solver* S, int* flag) {
   if (!S || !flag) return fail(QS_ERR_INVALID_ARG, "null argument");
   const Mesh* m = S->mesh;
-  QS_HIP(hipMemcpyAsync(flag, S->flag, sizeof(int), hipMemcpyDeviceToHost, stream_of(m)));
-  QS_HIP(hipStreamSynchronize(stream_of(m)));
+  QS_HIP(hipStreamSynchronize(stream_of(m)));   // every earlier copy into hflag has landed
+  QS_HIP(hipMemcpy(S->hflag, S->flag, sizeof(int), hipMemcpyDeviceToHost));
+  *flag = *S->hflag;
   QS_HIP(hipMemsetAsync(S->flag, 0, sizeof(int), stream_of(m)));
+  QS_HIP(hipStreamSynchronize(stream_of(m)));
+  *S->hflag = 0;
   return QS_OK;
 }
 

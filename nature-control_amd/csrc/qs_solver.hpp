@@ -45,7 +45,8 @@ struct qs_solver {
   double2* mix[7] = {};                              // mixed arrays [Ny][pitch]
   double* Ra = nullptr;                              // activity field or null
   double* w00 = nullptr;                             // W(0,0) of the stage
-  int* flag = nullptr;                               // non-finite flag
+  int* flag = nullptr;                               // non-finite flag (device)
+  int* hflag = nullptr;                              // its host copy (pinned), refreshed after each fused call
   qs::RunDesc* d_desc = nullptr;
   int2* d_work = nullptr;
   int nrow = 0, ncol = 0;

@@ -21,8 +21,9 @@ def rel(a, b):
 
 @pytest.mark.parametrize("N", [16, 32, 64, 128, 256, 512, 1024, 48, 96, 192, 384, 768])
 def test_wave_fft(N):
-    """Per-wavefront FFT: one wave's vectors (1024/N of them for N = 2^a,
-    768/N for N = 3 * 2^a), both directions, all twiddle quadrants."""
+    """Per-wavefront FFT: 1024/N vectors (768/N for N = 3 * 2^a), transformed
+    one wave's worth (WaveLay<N>::NV vectors) at a time, both directions, all
+    twiddle quadrants."""
     print(f"[gpu-test] wave fft N={N}", flush=True)
     m = Q.Mesh(64, 64, 2 * np.pi)
     rng = np.random.default_rng(N)
@@ -39,7 +40,9 @@ def test_wave_fft(N):
 @pytest.mark.parametrize("N", [32, 128, 256, 1024, 96, 768])
 def test_fused_phases(N):
     """Column prologue -> inverse set; row pass -> forward set; column stage 1
-    -> IFRK2 partial sums, against the numpy model of each kernel."""
+    -> IFRK2 partial sums and the stage's inverse set; row pass again; column
+    stage 2 -> the dealiased next state and its inverse set, against the numpy
+    model of each kernel."""
     print(f"[gpu-test] fused phases N={N}", flush=True)
     rng = np.random.default_rng(N)
     kx, ky = FR.tables(N, N)
@@ -68,4 +71,22 @@ def test_fused_phases(N):
     Mc = [g.debug_array(k).cpu().numpy() for k in range(7)]
     curr = [(old + dt * n) * IF for old, n, IF in zip(st, (Nw, Nr1, Nr2), (IFw, IFq, IFq))]
     for a, b in zip(Mc, FR.inverse_set(curr[0], curr[1], curr[2], kx, ky)):
+        assert rel(a, b) < 1e-12
+    # second right-hand side: row pass on the stage-1 inverse set, then column
+    # stage 2 (IFRK2.cuh:41-51, dealias, the next state over old, its inverse set)
+    new1 = [g.debug_array(10 + k).cpu().numpy() for k in range(3)]
+    g.debug_phase(1)
+    F2 = [g.debug_array(k).cpu().numpy() for k in range(7)]
+    for a, b in zip(F2, FR.row_pass(Mc, curr[0][0, 0], P, N, N, kx)):
+        assert rel(a, b) < 1e-12
+    g.debug_phase(3)
+    N2 = FR.nonlinear_from_forward_set(F2, kx, ky, N, N)
+    i = np.arange(N // 2 + 1)[None, :]
+    j = np.arange(N)[:, None]
+    mask = (i < (N // 2) * 2 // 3 + 1) & ((j < N // 3 + 1) | (N - j < N // 3))
+    nxt = [np.where(mask, a + 0.5 * dt * n, 0) for a, n in zip(new1, N2)]
+    for k in range(3):
+        assert rel(g.debug_array(7 + k).cpu().numpy(), nxt[k]) < 1e-12
+    M2 = [g.debug_array(k).cpu().numpy() for k in range(7)]
+    for a, b in zip(M2, FR.inverse_set(nxt[0], nxt[1], nxt[2], kx, ky)):
         assert rel(a, b) < 1e-12

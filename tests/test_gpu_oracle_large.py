@@ -1,0 +1,111 @@
+"""GPU parity at the benchmark sizes: the fused step at 1024^2 against the
+oracle's committed checksums, and 256^2 / 512^2 fused steps against the oracle
+run live.
+
+Reference step: InertialFriction/friction/friction.cu:212-260 with
+TimeIntegration/IFRK2.cuh:16-51 (RK4.cuh:12-74, IFE.cuh:15-25 for the other
+integrators).  Tolerance: 1e-12 relative L2 per field (per checksum part at
+1024^2: the norm, the |kx|, |ky| <= 24 mode block and the 2048 sampled
+coefficients, each as one vector).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import oracle_py as O  # noqa: E402
+import qsolver as Q  # noqa: E402
+from qsolver import synth  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CONFIG = dict(lam=1.0, Re=1.0, Er=1.0, Rf=0.1, alpha=1.0, DQ=0.05, aQ=1.0, cQ=1.0)
+KB = 24   # the fixture's mode block: rows 0..KB and Ny-KB..Ny-1, columns 0..KB
+
+
+def banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def qparams(dt):
+    return Q.params(lam=CONFIG["lam"], Re=CONFIG["Re"], Er=CONFIG["Er"], Rf=CONFIG["Rf"], alpha=CONFIG["alpha"],
+                    D_Q=CONFIG["DQ"], a_Q=CONFIG["aQ"], c_Q=CONFIG["cQ"], dt=dt)
+
+
+def oparams(dt):
+    return O.params_array(dt=dt, **CONFIG)
+
+
+def sample_index(Ny, Nxh, n=2048, seed=11):
+    """The fixture generator's sample positions (tests/golden/make_golden_1024.py)."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, Ny, n), rng.integers(0, Nxh, n)
+
+
+@pytest.mark.parametrize("tag,dt,nsteps", [("dt5e-4_10", 5e-4, 10), ("dt1e-4_100", 1e-4, 100)])
+def test_1024_fused_step_vs_oracle_fixture(tag, dt, nsteps):
+    """The benchmark workload (1024^2, band-limited Q from seed 2, rms 0.05,
+    omega = 0) through the fused qs_step_ifrk2 against the oracle's checksums
+    of the same run: 10 steps at the benchmark's dt = 5e-4 (the scheme leaves
+    its stability region near step 75 at this dt) and 100 steps at dt = 1e-4."""
+    banner(f"1024^2 fused IFRK2 {tag} vs oracle fixture")
+    g = np.load(os.path.join(HERE, "golden", "ifrk2_1024.npz"))
+    N = 1024
+    om = O.Mesh(N, N, 2 * np.pi)
+    q11, q12 = synth.bandlimited_q(N, N, seed=2)
+    st = [np.zeros(om.spec_shape, complex), om.fwd(q11), om.fwd(q12)]
+    m = Q.Mesh(N, N, 2 * np.pi)
+    s = Q.Solver(m, qparams(dt))
+    s.set_state(*(torch.from_numpy(a).cuda() for a in st))
+    s.step(nsteps)
+    got = [t.cpu().numpy() for t in s.get_state()]
+    assert not s.nonfinite()
+    s.close()
+    rows = np.r_[0:KB + 1, N - KB:N]
+    jj, ii = sample_index(N, N // 2 + 1)
+    for name, a in zip(("w", "r1", "r2"), got):
+        norm = np.linalg.norm(a)
+        assert abs(norm - g[f"{tag}_{name}_norm"]) <= 1e-12 * g[f"{tag}_{name}_norm"], name
+        assert rel_l2(a[np.ix_(rows, np.arange(KB + 1))], g[f"{tag}_{name}_block"]) < 1e-12, name
+        assert rel_l2(a[jj, ii], g[f"{tag}_{name}_sample"]) < 1e-12, name
+
+
+def init_state(N, kind, seed):
+    om = O.Mesh(N, N, 2 * np.pi)
+    q11, q12 = synth.uniform_q(N, N, seed) if kind == "uniform" else synth.bandlimited_q(N, N, seed)
+    return [np.zeros(om.spec_shape, complex), om.fwd(q11), om.fwd(q12)]
+
+
+@pytest.mark.parametrize("method,N,nsteps,kind", [("ifrk2", 256, 10, "bandlimited"),
+                                                  ("ifrk2", 512, 10, "bandlimited"),
+                                                  ("ifrk2", 512, 20, "uniform"),
+                                                  ("rk4", 512, 5, "bandlimited"),
+                                                  ("ife", 512, 10, "uniform"),
+                                                  ("rk4", 256, 10, "uniform")])
+def test_fused_256_512_vs_oracle(method, N, nsteps, kind):
+    """Square 256^2 (8 elements per lane) and 512^2 fused steps against the
+    oracle run on the same seeded state."""
+    banner(f"fused {method} {N}^2 steps={nsteps} init={kind} vs oracle")
+    st = init_state(N, kind, seed=4)
+    dt = 1e-3
+    om = O.Mesh(N, N, 2 * np.pi)
+    s = O.Solver(om, oparams(dt))
+    s.set_state(*st)
+    s.step(nsteps, {"ifrk2": O.IFRK2, "rk4": O.RK4, "ife": O.IFE}[method])
+    ref = s.get_state()
+    m = Q.Mesh(N, N, 2 * np.pi)
+    g = Q.Solver(m, qparams(dt))
+    g.set_state(*(torch.from_numpy(a).cuda() for a in st))
+    {"ifrk2": g.step, "rk4": g.step_rk4, "ife": g.step_ife}[method](nsteps)
+    got = [t.cpu().numpy() for t in g.get_state()]
+    assert not g.nonfinite()
+    g.close()
+    for a, b in zip(got, ref):
+        assert rel_l2(a, b) < 1e-12
