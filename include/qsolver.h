@@ -210,7 +210,11 @@ int qs_step_ifrk2_unfused(qs_solver* s, int nsteps);
 /* The reference driver's RK4 step loop itself (friction.cu:212-260 with
  * TimeIntegration/RK4.cuh:12-74) through the per-op entry points. */
 int qs_step_rk4_unfused(qs_solver* s, int nsteps);
-/* 1 if a non-finite value reached the state since the last query. */
+/* 1 if a non-finite value reached the state since the last query (the
+ * reference checks the snapshot's S for NaN, friction.cu:280).  Synchronises,
+ * then clears the flag.  While it is set, qs_step_ifrk2 / _rk4 / _ife return
+ * QS_ERR_NONFINITE without running (they notice it without synchronising,
+ * from an asynchronous copy each of them ends with). */
 int qs_solver_nonfinite(qs_solver* s, int* flag);
 /* No reference counterpart (the reference has no timers).  While enabled,
  * every kernel of qs_step_ifrk2 is bracketed by HIP events on the Mesh's
@@ -233,6 +237,21 @@ int qs_solver_debug_array(qs_solver* s, int which, Qcomp* out);
  * unnormalised) on 1024/n (resp. 768/n) contiguous vectors, transformed one
  * wave's worth of vectors at a time. */
 int qs_debug_wave_fft(const Mesh* mesh, int n, int sign, const Qcomp* in, Qcomp* out);
+
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops): nsteps fused IFRK2
+ * steps, the kernels and results of qs_solver_set_state + qs_step_ifrk2, on
+ * the caller's state spectra w, r1, r2 (device, reference layout, updated in
+ * place), with parameters *p and activity field Ra (device Nx*Ny, or NULL for
+ * the uniform p->alpha), on the Mesh's stream.  nonfinite: device int set to
+ * 1 (0) if a non-finite value reached (did not reach) the state, or NULL.
+ * The working arrays live in one fixed device array per GPU (21 MB, held
+ * until the process ends): meshes of at most 512 x 257 spectral entries per
+ * array (512 x 512, 1024 x 256, ...), larger ones are QS_ERR_UNSUPPORTED_SIZE.
+ * Calls no allocating routine.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (several streams or threads) race. */
+int qs_step_ifrk2_noalloc(const Mesh* mesh, const qs_params* p, const Qreal* Ra, Qcomp* w, Qcomp* r1, Qcomp* r2,
+                          int nsteps, int* nonfinite);
 
 /* ---- discrete adjoint of the IFRK2 step ----------------------------------
  * No reference counterpart (the reference has no adjoint).  Exact transpose of

@@ -147,6 +147,7 @@ _SIGS = {
     "qs_solver_profile_read": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I)],
     "qs_synth_uniform": [ctypes.c_uint64, _D, _D, ctypes.c_size_t, _P],
     "qs_step_rk4": [_P, _I],
+    "qs_step_ifrk2_noalloc": [_P, ctypes.POINTER(QsParams), _P, _P, _P, _P, _I, _P],
     "qs_step_ife": [_P, _I],
     "qs_adj_forward": [_P, _I, _I, _P],
     "qs_adj_backward": [_P] * 9,
