@@ -57,6 +57,8 @@ def parse():
                          "fused and through the per-op calls")
     ap.add_argument("--driver-grid-only", action="store_true",
                     help="only run the reference-driver configuration and print its line")
+    ap.add_argument("--config1", type=int, default=1,
+                    help="also time the 64^2 x 2000-step CPU-path workload in full (oracle and GPU)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times "
                          "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -234,8 +236,9 @@ def batched_adjoint_bench(nenv=32, N=512, n=500, dt=5e-4):
     (splitmix64 seed 100 + e), omega = 0 (a band-limited rms-0.05 init leaves
     the scheme's stability region at this dt: most such runs turn non-finite
     before step 500).  grad-steps/s = 500 / (forward with checkpoints + losses +
-    reverse sweep).  Peak HBM: the adjoint's own device bytes and the device
-    memory in use after the reverse sweep (hipMemGetInfo)."""
+    reverse sweep).  Peak HBM: the adjoint's own device bytes, and the device
+    memory in use (hipMemGetInfo) sampled by a watcher thread during the
+    sweeps (high-water mark) and after them."""
     import math
     import numpy as np
     import torch
@@ -264,11 +267,26 @@ def batched_adjoint_bench(nenv=32, N=512, n=500, dt=5e-4):
         gt, _ = e.adj_backward(n)
         torch.cuda.synchronize()
         return losses, gt
+    import threading
+    peak = [0]
+    stop = threading.Event()
+
+    def watch():
+        # hipMemGetInfo high-water mark while the sweeps run (the C-ABI calls
+        # release the interpreter lock, so this thread samples during them)
+        while not stop.is_set():
+            f, t = torch.cuda.mem_get_info()
+            peak[0] = max(peak[0], t - f)
+            time.sleep(0.002)
+    wt = threading.Thread(target=watch, daemon=True)
+    wt.start()
     grad_once()                                   # allocation, warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     losses, gt = grad_once()
     tg = time.perf_counter() - t0
+    stop.set()
+    wt.join()
     free1, _ = torch.cuda.mem_get_info()
     out = {"workload": f"{nenv} envs x {N}^2 FP64 IFRK2, dt={dt}, {n} steps, per-env per-step activity "
                        f"U[0.8,1.2], L = sum_e 1/2 mean(u^2+v^2), checkpoint every {k} steps, "
@@ -276,6 +294,8 @@ def batched_adjoint_bench(nenv=32, N=512, n=500, dt=5e-4):
            "grad_steps_per_s": round(n / tg, 2), "env_grad_steps_per_s": round(nenv * n / tg, 1),
            "grad_s": round(tg, 3), "adjoint_device_bytes": e.adj_memory(),
            "device_bytes_in_use": int(total - free1), "device_bytes_before": int(total - free0),
+           "peak_device_bytes": int(peak[0]),
+           "peak_note": "hipMemGetInfo high-water mark sampled every 2 ms during the warm-up and timed sweeps",
            "finite": bool(np.all(np.isfinite(losses)) and bool(torch.isfinite(gt).all())),
            "loss_sum": float(np.sum(losses))}
     e.close()
@@ -290,13 +310,41 @@ def ctypes_solver(e, r):
     return out
 
 
+def host_info():
+    """The host's CPU as this process sees it: model, logical CPUs, the CPUs it
+    may run on, and the OpenMP thread budget the environment sets."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: every CPU this process may run on, within
+    the OpenMP budget the environment sets (on a GPU box, the CPU share of one
+    GPU)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
 def cpu_baseline(N, dt, nsteps, q11, q12):
     """The FP64 CPU oracle on this host's cores, a bounded sample of the same
-    workload (nsteps IFRK2 steps from the same initial state)."""
+    workload (nsteps IFRK2 steps from the same initial state), on all the
+    threads cpu_threads() allows and on one core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle_py as O
-    cores = min(16, os.cpu_count() or 1)
+    cores = cpu_threads()
     O.set_threads(cores)
     m = O.Mesh(N, N, 2 * np.pi)
     p = O.params_array(lam=CONFIG["lam"], Re=CONFIG["Re"], Er=CONFIG["Er"], Rf=CONFIG["Rf"],
@@ -320,8 +368,60 @@ def cpu_baseline(N, dt, nsteps, q11, q12):
     return {"value": nsteps / el, "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": f"{nsteps} IFRK2 steps at {N}^2 FP64 (same initial state), "
                       f"{el:.2f} s wall on {threads} OpenMP threads",
+            "build": "g++ -O3 -march=x86-64-v3 -fopenmp (oracle/Makefile)",
+            "host": host_info(),
             "single_core": {"value": n1 / el1, "unit": "steps/s", "cores": 1,
                             "sample": f"{n1} IFRK2 steps from the same initial state, {el1:.2f} s wall"}}
+
+
+def config1_bench(nsteps=2000):
+    """The CPU-path workload timed in full: 64^2, L = 2 pi, FP64, dt = 1e-3,
+    2000 IFRK2 steps, lambda = Re = Er = 1, Rf = 0.1, alpha = 1, D_Q = 0.05,
+    a_Q = c_Q = 1; q11, q12 uniform [-0.01, 0.01] (splitmix64 seed 1), omega =
+    0.  The oracle on one core and on cpu_threads() threads, and the fused GPU
+    step, each over the whole 2000 steps; the final states agree."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import torch
+    import oracle_py as O
+    import qsolver as Q
+    from qsolver import synth
+    N, dt = 64, 1e-3
+    m = O.Mesh(N, N, 2 * np.pi)
+    q11, q12 = synth.uniform_q(N, N, seed=1)
+    st = [np.zeros(m.spec_shape, complex), m.fwd(q11), m.fwd(q12)]
+    p = O.params_array(lam=CONFIG["lam"], Re=CONFIG["Re"], Er=CONFIG["Er"], Rf=CONFIG["Rf"],
+                       alpha=CONFIG["alpha"], DQ=CONFIG["D_Q"], aQ=CONFIG["a_Q"], cQ=CONFIG["c_Q"], dt=dt)
+    out = {"workload": "64x64 FP64 IFRK2, L=2pi, dt=1e-3, 2000 steps, uniform Q +-0.01 (seed 1), omega=0",
+           "steps": nsteps}
+    ref = None
+    for label, th in (("cpu_1_core", 1), ("cpu_all_threads", cpu_threads())):
+        O.set_threads(th)
+        s = O.Solver(m, p)
+        s.set_state(*st)
+        t0 = time.perf_counter()
+        s.step(nsteps, O.IFRK2)
+        el = time.perf_counter() - t0
+        ref = s.get_state()
+        out[label] = {"steps_per_s": round(nsteps / el, 1), "threads": th, "wall_s": round(el, 3)}
+    mesh = Q.Mesh(N, N, 2 * np.pi)
+    g = Q.Solver(mesh, Q.params(dt=dt, **CONFIG))
+    dev = [torch.from_numpy(a).cuda() for a in st]
+    g.set_state(*dev)
+    g.step(20)                      # warm-up
+    g.set_state(*dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.step(nsteps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    got = [t.cpu().numpy() for t in g.get_state()]
+    err = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(got, ref))
+    out["gpu_fused"] = {"steps_per_s": round(nsteps / el, 1), "wall_s": round(el, 4),
+                        "rel_l2_vs_oracle_after_2000": err}
+    out["timing"] = "wall clock around the whole 2000-step loop (GPU: between synchronisations)"
+    g.close()
+    return out
 
 
 def driver_grid_bench(fused_steps=100, unfused_steps=10):
@@ -480,6 +580,7 @@ def main():
     ens = ensemble_bench() if args.ensemble else None
     badj = batched_adjoint_bench() if args.batched_adjoint else None
     drv = driver_grid_bench() if args.driver_grid else None
+    cfg1 = config1_bench() if args.config1 else None
     cpu = None
     if args.cpu_steps > 0:
         cpu = cpu_baseline(N, dt, args.cpu_steps, q11, q12)
@@ -518,6 +619,7 @@ def main():
         "ensemble": ens,
         "batched_adjoint": badj,
         "driver_grid": drv,
+        "config1": cfg1,
     }
     print(json.dumps(out), flush=True)
 
