@@ -336,6 +336,7 @@ int reverse_segment(AdjGroup* G, int seg, hipStream_t st) {
 // final state); the last segment records its stages.  alpha_t: device
 // [R][nsteps] or null.
 int group_forward(AdjGroup* G, int nsteps, int k, const double* alpha_t) {
+  Range rg("adjoint forward");
   QS_TRY(group_prepare(G, nsteps, k));
   const int R = G->R;
   G->nsteps = nsteps;
@@ -373,6 +374,7 @@ int group_forward(AdjGroup* G, int nsteps, int k, const double* alpha_t) {
 
 // Reverse sweep.  The seeds dL/d(final state) are in every run's lam_.
 int group_backward(AdjGroup* G) {
+  Range rg("adjoint backward");
   const int nsteps = G->nsteps, nseg = nseg_of(G), nbuf = G->nbuf;
   const Mesh* m = mesh_of(G);
   const hipStream_t st = stream_g(G);
@@ -389,6 +391,7 @@ int group_backward(AdjGroup* G) {
   }
   bool rec_pending[2] = {false, false}, rev_pending[2] = {false, false};
   auto recompute = [&](int seg) -> int {
+    Range r("adjoint recompute segment %d", seg);
     const int b = seg % nbuf;
     if (nbuf == 2 && rev_pending[b]) QS_HIP(hipStreamWaitEvent(side, G->rev_done[b], 0));
     for (auto* S : G->runs)
@@ -401,6 +404,7 @@ int group_backward(AdjGroup* G) {
     return QS_OK;
   };
   auto reverse = [&](int seg) -> int {
+    Range r("adjoint reverse segment %d", seg);
     const int b = seg % nbuf;
     if (nbuf == 2 && rec_pending[b]) QS_HIP(hipStreamWaitEvent(st, G->rec_done[b], 0));
     QS_TRY(reverse_segment(G, seg, st));

@@ -315,6 +315,7 @@ int qs_ensemble_set_activity(qs_ensemble* E, int r, const Qreal* Ra) {
 int qs_ensemble_step(qs_ensemble* E, int nsteps) {
   if (!E) return fail(QS_ERR_INVALID_ARG, "null ensemble");
   if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
+  Range rg("qs_ensemble_step");
   return ensemble_steps(E, nsteps);
 }
 

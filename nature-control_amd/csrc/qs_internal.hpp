@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdio>
 #include <string>
@@ -31,6 +32,21 @@ int fail(int status, const char* fmt, ...);
     int s_ = (expr);              \
     if (s_ != QS_OK) return s_;   \
   } while (0)
+
+// Host range for rocprofv3 --marker-trace (roctx): the span of a call on the
+// host (launch and enqueue), nested as the calls nest.  Without a profiler
+// attached a push / pop is a call into an idle stub.
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  Range(const char* fmt, int a) {
+    char b[64];
+    std::snprintf(b, sizeof b, fmt, a);
+    roctxRangePushA(b);
+  }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 // Mesh sides the per-op transforms support: 2^a in [16, 2048] and 3 * 2^a in

@@ -318,13 +318,20 @@ int qs_step_ifrk2(qs_solver* S, int nsteps) {
   if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
   if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
   QS_TRY(nonfinite_gate(S));
+  Range rg("qs_step_ifrk2");
   for (int n = 0; n < nsteps; ++n) {
+    Range rs("ifrk2 step %d", n);
     if (!S->mixed_valid) {
+      Range rp("column prologue");
       QS_TRY(col_stage(S, MODE_PRO));
       S->mixed_valid = true;
     }
-    QS_TRY(row_rhs(S));
-    QS_TRY(col_stage(S, MODE_S1));
+    {
+      Range r1("ifrk2 stage 1");
+      QS_TRY(row_rhs(S));
+      QS_TRY(col_stage(S, MODE_S1));
+    }
+    Range r2("ifrk2 stage 2");
     QS_TRY(row_rhs(S));
     QS_TRY(col_stage(S, MODE_S2));
   }
@@ -338,12 +345,16 @@ int qs_step_rk4(qs_solver* S, int nsteps) {
   if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
   if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
   QS_TRY(nonfinite_gate(S));
+  Range rg("qs_step_rk4");
   for (int n = 0; n < nsteps; ++n) {
+    Range rs("rk4 step %d", n);
     if (!S->mixed_valid) {
+      Range rp("column prologue");
       QS_TRY(col_stage(S, MODE_PRO_R4));
       S->mixed_valid = true;
     }
     for (int mode : {MODE_R1, MODE_R2, MODE_R3, MODE_R4}) {
+      Range rq("rk4 stage %d", mode - MODE_R1 + 1);
       QS_TRY(row_rhs(S));
       QS_TRY(col_stage(S, mode));
     }
@@ -357,8 +368,11 @@ int qs_step_ife(qs_solver* S, int nsteps) {
   if (!S) return fail(QS_ERR_INVALID_ARG, "null solver");
   if (nsteps < 0) return fail(QS_ERR_INVALID_ARG, "nsteps < 0");
   QS_TRY(nonfinite_gate(S));
+  Range rg("qs_step_ife");
   for (int n = 0; n < nsteps; ++n) {
+    Range rs("ife step %d", n);
     if (!S->mixed_valid) {
+      Range rp("column prologue");
       QS_TRY(col_stage(S, MODE_PRO_E));
       S->mixed_valid = true;
     }
