@@ -250,12 +250,8 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
   const double dt = a.dt;
   auto kyj = [&](int j) { return a.dky * (j < NY / 2 + 1 ? j : j - NY); };
   auto keep = [&](int j) { return (i < ((a.Nxh - 1) * 2 / 3 + 1)) && ((j < NY / 3 + 1) || ((NY - j) < NY / 3)); };
-  auto IFof = [&](int j) {
-    const double ky = kyj(j);
-    const double k2 = kx * kx + ky * ky;
-    const double Lk = FLD == 0 ? 1.0 / a.Re * (-1.0 * k2 - a.Rf) : (-1.0 * a.DQ * k2 + a.aQ);
-    return exp(Lk * dt);
-  };
+  const IFac ifc(a, FLD, kx, NY);   // the forward's integrating factor, exp(L dt)
+  auto IFof = [&](int j) { return ifc.IF(j); };
 
   // W(0,0) enters every row's dy u + dx v at bin 0: its adjoint is the sum of
   // the row adjoints' parts (fixed-order block reduction; thread 0 owns the

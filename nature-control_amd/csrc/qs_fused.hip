@@ -222,13 +222,8 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       }
     }
     // ---- stage update (state tiles read and rewritten in LDS) ---------------
-    // N_X is in st.  Per element: L_k, IF = exp(L dt), IFh = exp(L dt / 2)
-    // (friction/Qsolver.cuh:20-54 with the D_Q, a_Q coefficients).
-    auto lin = [&](int j) {
-      const double ky = kyj(j);
-      const double k2 = kx * kx + ky * ky;
-      return FLD == 0 ? 1.0 / a.Re * (-1.0 * k2 - a.Rf) : (-1.0 * a.DQ * k2 + a.aQ);
-    };
+    // N_X is in st.  Per element: IF = exp(L dt), IFh = exp(L dt / 2) (IFac).
+    const IFac ifc(a, FLD, kx, NY);
     auto keep = [&](int j) {   // dealias mask, Qsolver.cuh:314-322
       return (i < ((a.Nxh - 1) * 2 / 3 + 1)) && ((j < NY / 3 + 1) || ((NY - j) < NY / 3));
     };
@@ -246,23 +241,23 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
         double2* sl = vb + swz(j);
         const double2 in = D.at(j, cl);
         const double2 n = st[m];
-        const double Lk = lin(j);
-        const double IF = exp(Lk * a.dt);
         const double2 an = make_double2(n.x * a.dt, n.y * a.dt);
         if (MODE == MODE_S1) {
+          const double IF = ifc.IF(j);
           const double2 old = in;
           const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // IFRK2.cuh:16-26
           const double h = 1.0 / 2.0 * IF;
           *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // IFRK2.cuh:28-39
           st[m] = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);      // curr (IFh = IF)
         } else if (MODE == MODE_R1) {
+          const double IF = ifc.IF(j), IFh = ifc.IFh(j);
           const double2 old = in;
-          const double IFh = exp(Lk * a.dt * 0.5);
           const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // RK4.cuh:12-22
           const double h = 1.0 / 6.0 * IF;
           *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // RK4.cuh:25-36
           st[m] = make_double2((old.x + an.x / 2.0) * IFh, (old.y + an.y / 2.0) * IFh);
         } else if (MODE == MODE_E) {
+          const double IF = ifc.IF(j);
           const double2 old = in;                                                // IFE.cuh:15-25
           double2 r = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);
           const double cut = keep(j) ? 1.0 : 0.0;
@@ -291,7 +286,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
         const double2 nw = D.at(j, cl);
-        const double IFh = exp(lin(j) * a.dt * 0.5);
+        const double IFh = ifc.IFh(j);
         const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
         const double h = 1.0 / 3.0 * IFh;                                          // RK4.cuh:37-61
         *sl = make_double2(nw.x + h * bn.x, nw.y + h * bn.y);
@@ -303,13 +298,12 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         const double2 old = D.at(j, cl);
-        const double Lk = lin(j);
-        const double IFh = exp(Lk * a.dt * 0.5);
+        const double IFh = ifc.IFh(j);
         const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
         if (MODE == MODE_R2) {
           st[m] = make_double2(old.x * IFh + bn.x / 2.0, old.y * IFh + bn.y / 2.0);
         } else {
-          const double IF = exp(Lk * a.dt);
+          const double IF = ifc.IF(j);
           st[m] = make_double2(old.x * IF + bn.x * IFh, old.y * IF + bn.y * IFh);
         }
       }
@@ -491,6 +485,19 @@ int launch_col_n(int mode, const RunDesc* runs, const int2* work, int nwork, int
 }
 
 }  // namespace
+
+namespace {
+__global__ __launch_bounds__(256) void k_ifac_rows(double* ify, int Ny, double dky, double Re, double DQ, double dt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < Ny) ifac_row(j, Ny, dky, Re, DQ, dt, ify);
+}
+}  // namespace
+
+int ifac_rows_launch(double* ify, int Ny, double dky, double Re, double DQ, double dt, hipStream_t st) {
+  hipLaunchKernelGGL(k_ifac_rows, dim3((Ny + 255) / 256), dim3(256), 0, st, ify, Ny, dky, Re, DQ, dt);
+  QS_CHECK_LAUNCH();
+  return QS_OK;
+}
 
 int row_jobs(const RunDesc& a) { return (a.Ny + wave_nv(a.Nx) - 1) / wave_nv(a.Nx); }
 int col_jobs(const RunDesc& a) {

@@ -1,6 +1,8 @@
 // qs_fused.hpp -- descriptors and launchers of the fused IFRK2 kernels.
 #pragma once
 
+#include <cmath>
+
 #include "qs_internal.hpp"
 
 namespace qs {
@@ -33,6 +35,7 @@ struct RunDesc {
   double* w00;                    // W(0,0) of the stage the row pass reads
   double* w00o;                   // W(0,0) of the stage the column kernel writes
   int* flag;                      // non-finite flag
+  const double* ify;              // [4][Ny] row factors of the integrating factors (IFac)
   double dkx, dky;                // 2 pi / Lx, 2 pi / Ly: kx(i) = dkx i, ky(j) = dky (j or j - Ny)
   double alpha, lambda, DQ, aQ, cQ, s_ErRe, scale, dt, Re, Rf;
   int advect_w;
@@ -75,5 +78,21 @@ int loss_launch(const Mesh* m, int kind, const double2* W, int pin, double* loss
 int wave_fft_test(int n, int sign, const double2* in, double2* out, const double2* tw1024, hipStream_t st);
 
 inline int pitch_of(int Nxh) { return (Nxh + 7) & ~7; }
+
+// The integrating-factor row factors of IFac (row q of [4][Ny]: 0 w with
+// h = dt, 1 Q with dt, 2 w with dt/2, 3 Q with dt/2): exp(c2 ky(j)^2 h), c2 =
+// -1/Re for w, -D_Q for Q.  One device routine for every table, so that runs
+// built by different paths compute identical factors.
+__device__ __forceinline__ void ifac_row(int j, int Ny, double dky, double Re, double DQ, double dt, double* ify) {
+  const double ky = dky * (j < Ny / 2 + 1 ? j : j - Ny);
+  const double cw = -1.0 / Re * ky * ky, cq = -DQ * ky * ky;
+  ify[j] = exp(cw * dt);
+  ify[Ny + j] = exp(cq * dt);
+  ify[2 * Ny + j] = exp(cw * (dt * 0.5));
+  ify[3 * Ny + j] = exp(cq * (dt * 0.5));
+}
+// Fills a solver's table on stream st.
+int ifac_rows_launch(double* ify, int Ny, double dky, double Re, double DQ, double dt, hipStream_t st);
+
 
 }  // namespace qs

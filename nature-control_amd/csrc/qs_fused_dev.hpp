@@ -2,6 +2,8 @@
 // (qs_fused.hip) and their discrete adjoint (qs_adjoint.hip).
 #pragma once
 
+#include <cmath>
+
 #include "qs_fused.hpp"
 
 namespace qs {
@@ -108,6 +110,29 @@ __device__ __forceinline__ double recip(double x) {
   return r * (2.0 - x * r);
 }
 
+// Integrating factors exp(L dt) and exp(L dt / 2) of the column kernels
+// (friction/Qsolver.cuh:20-54 with the D_Q, a_Q coefficients), separably:
+// L = c2 (kx^2 + ky^2) + c0 with c2 = -1/Re, c0 = -Rf/Re for w and c2 = -D_Q,
+// c0 = a_Q for r1, r2, so exp(L h) = exp((c2 kx^2 + c0) h) exp(c2 ky^2 h): one
+// exp per lane (its column) per factor, and the row factor from the run's
+// table a.ify (row q of [4][Ny]: 0 w with h = dt, 1 Q dt, 2 w dt/2, 3 Q dt/2,
+// built by the solver).  Within a few ulp of exp(L h).
+struct IFac {
+  double fx, fxh;
+  const double* fy;
+  const double* fyh;
+  __device__ __forceinline__ IFac(const RunDesc& a, int FLD, double kx, int NY) {
+    const double c2 = FLD == 0 ? -1.0 / a.Re : -a.DQ;
+    const double c0 = FLD == 0 ? -a.Rf / a.Re : a.aQ;
+    const double l = c2 * kx * kx + c0;
+    fx = exp(l * a.dt);
+    fxh = exp(l * (a.dt * 0.5));
+    fy = a.ify + (FLD == 0 ? 0 : NY);
+    fyh = a.ify + (FLD == 0 ? 2 * NY : 3 * NY);
+  }
+  __device__ __forceinline__ double IF(int j) const { return fx * gload(fy + j); }
+  __device__ __forceinline__ double IFh(int j) const { return fxh * gload(fyh + j); }
+};
 // Row staging: rows row0 .. row0+NV-1 of the 7 arrays arr[0..6] (x-spectral,
 // H = N/2+1 bins each) into sm[t*RH + vec*H + k].  RH = NV*H <= 576 < 2*384:
 // two slots per thread and array, all 14 loads in flight before the first LDS

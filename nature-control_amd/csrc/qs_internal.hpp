@@ -455,31 +455,42 @@ __device__ __forceinline__ void wave_sync() {
 // of the first Stockham write across the 64 lanes).
 __device__ __forceinline__ int swz(int e) { return e + (e >> 4); }
 
-// ---------------------------------------------------------------------------
-// Per-wavefront FFT of NV independent length-N vectors (WaveLay<N>).  Lane
-// (lp, g) with lp < LPC holds elements lp + LPC*m, m < E, of vector g in v[];
-// on return it holds the transform at the same positions.  Radix-E first pass
-// in registers, then Stockham passes of radix min(RMAX, N/P) through `buf`
-// (this lane's vector in its wave's LDS buffer, VS entries).  Only wave-level
-// ordering is needed: no workgroup barrier.
-// ---------------------------------------------------------------------------
-template <int N, int P, int SIGN, class TW>
-__device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, int lp, const TW& tw) {
+// One Stockham pass P of the per-wave FFT (radix R = min(RMAX, N/P), Q = N/R
+// butterflies per vector, BPL per lane) in its three steps: the lane's E
+// inputs from `buf`, twiddles and radix-R butterflies in registers, outputs
+// to `buf` (or into v after the last pass).  t holds butterfly q's inputs at
+// t[q R + r].
+template <int N, int P>
+struct PassShape {
   using L = WaveLay<N>;
-  constexpr int LPC = L::LPC;
-  constexpr int R = (N / P >= L::RMAX) ? L::RMAX : N / P;
-  constexpr int Q = N / R;
-  constexpr int BPL = Q / LPC;          // butterflies per lane
+  static constexpr int R = (N / P >= L::RMAX) ? L::RMAX : N / P;
+  static constexpr int Q = N / R;
+  static constexpr int BPL = Q / L::LPC;   // butterflies per lane
+  static constexpr bool LAST = (P * R == N);
+  static constexpr int STEP = N / (P * R);
   static_assert(BPL * R == L::E && N % (P * R) == 0, "pass shape");
-  constexpr bool LAST = (P * R == N);
-  constexpr int STEP = N / (P * R);
-  double2 t[BPL][R];
+};
+
+template <int N, int P>
+__device__ __forceinline__ void pass_load(double2 (&t)[16], const double2* buf, int lp) {
+  using S = PassShape<N, P>;
 #pragma unroll
-  for (int q = 0; q < BPL; ++q) {
-    const int b = lp + LPC * q;
+  for (int q = 0; q < S::BPL; ++q) {
+    const int b = lp + WaveLay<N>::LPC * q;
+#pragma unroll
+    for (int r = 0; r < S::R; ++r) t[q * S::R + r] = buf[swz(b + r * S::Q)];
+  }
+}
+
+template <int N, int P, int SIGN, class TW>
+__device__ __forceinline__ void pass_compute(double2 (&t)[16], int lp, const TW& tw) {
+  using S = PassShape<N, P>;
+  constexpr int R = S::R, STEP = S::STEP;
+#pragma unroll
+  for (int q = 0; q < S::BPL; ++q) {
+    const int b = lp + WaveLay<N>::LPC * q;
     const int k = (int)((unsigned)b % (unsigned)P);   // b >= 0: an AND for P = 2^a
-#pragma unroll
-    for (int r = 0; r < R; ++r) t[q][r] = buf[swz(b + r * Q)];
+    double2* tq = t + q * R;
     // Twiddles w^r, w = W^(k STEP).  With a per-pass table (TwT): one LDS read
     // each.  Otherwise the powers w^1, w^2, w^4, w^8 come from the quarter
     // table and every other w^r is the product of the powers of its set bits
@@ -492,7 +503,7 @@ __device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, in
       for (int r = 1; r < R; ++r) {
         double2 w = tp[r - 1];
         if (SIGN > 0) w.y = -w.y;
-        t[q][r] = cmul(t[q][r], w);
+        tq[r] = cmul(tq[r], w);
       }
     } else {
       constexpr int NPOW = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : 1));
@@ -510,27 +521,97 @@ __device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, in
 #pragma unroll
         for (int e = lo + 1; e < NPOW; ++e)
           if (r & (1 << e)) w = cmul(w, wp[e]);
-        t[q][r] = cmul(t[q][r], w);
+        tq[r] = cmul(tq[r], w);
       }
     }
-    dft_any<R, SIGN>(t[q]);
+    dft_any<R, SIGN>(tq);
   }
-  if constexpr (LAST) {
+}
+
+template <int N, int P>
+__device__ __forceinline__ void pass_store(const double2 (&t)[16], double2* buf, int lp) {
+  using S = PassShape<N, P>;
 #pragma unroll
-    for (int q = 0; q < BPL; ++q)
+  for (int q = 0; q < S::BPL; ++q) {
+    const int b = lp + WaveLay<N>::LPC * q;
+    const int k = (int)((unsigned)b % (unsigned)P);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q + r * BPL] = t[q][r];
+    for (int r = 0; r < S::R; ++r) buf[swz((b - k) * S::R + k + r * P)] = t[q * S::R + r];
+  }
+}
+
+template <int N, int P>
+__device__ __forceinline__ void pass_final(const double2 (&t)[16], double2 (&v)[16]) {
+  using S = PassShape<N, P>;
+#pragma unroll
+  for (int q = 0; q < S::BPL; ++q)
+#pragma unroll
+    for (int r = 0; r < S::R; ++r) v[q + r * S::BPL] = t[q * S::R + r];
+}
+
+// ---------------------------------------------------------------------------
+// Per-wavefront FFT of NV independent length-N vectors (WaveLay<N>).  Lane
+// (lp, g) with lp < LPC holds elements lp + LPC*m, m < E, of vector g in v[];
+// on return it holds the transform at the same positions.  Radix-E first pass
+// in registers, then Stockham passes of radix min(RMAX, N/P) through `buf`
+// (this lane's vector in its wave's LDS buffer, VS entries).  Only wave-level
+// ordering is needed: no workgroup barrier.
+// ---------------------------------------------------------------------------
+template <int N, int P, int SIGN, class TW>
+__device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, int lp, const TW& tw) {
+  double2 t[16];
+  pass_load<N, P>(t, buf, lp);
+  pass_compute<N, P, SIGN>(t, lp, tw);
+  if constexpr (PassShape<N, P>::LAST) {
+    pass_final<N, P>(t, v);
   } else {
     wave_sync();
-#pragma unroll
-    for (int q = 0; q < BPL; ++q) {
-      const int b = lp + LPC * q;
-      const int k = (int)((unsigned)b % (unsigned)P);   // b >= 0: an AND for P = 2^a
-#pragma unroll
-      for (int r = 0; r < R; ++r) buf[swz((b - k) * R + k + r * P)] = t[q][r];
-    }
+    pass_store<N, P>(t, buf, lp);
     wave_sync();
-    wave_fft_pass<N, P * R, SIGN>(v, buf, lp, tw);
+    wave_fft_pass<N, P * PassShape<N, P>::R, SIGN>(v, buf, lp, tw);
+  }
+}
+
+// Cross-lane swaps of gfx950 on a double2 (four dwords):
+//   pl32: lanes 32-63 of a <-> lanes 0-31 of b   (v_permlane32_swap)
+//   pl16: lanes 16-31 of a <-> lanes 0-15 of b, lanes 48-63 of a <-> 32-47 of b
+//         (v_permlane16_swap)
+__device__ __forceinline__ void swap_dw(double& a, double& b, bool rows32) {
+  const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  unsigned alo = (unsigned)ua, ahi = (unsigned)(ua >> 32), blo = (unsigned)ub, bhi = (unsigned)(ub >> 32);
+  if (rows32) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+    alo = lo[0]; blo = lo[1]; ahi = hi[0]; bhi = hi[1];
+  } else {
+    const auto lo = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+    alo = lo[0]; blo = lo[1]; ahi = hi[0]; bhi = hi[1];
+  }
+  a = __longlong_as_double((long long)(((unsigned long long)ahi << 32) | alo));
+  b = __longlong_as_double((long long)(((unsigned long long)bhi << 32) | blo));
+}
+__device__ __forceinline__ void swap_c(double2& a, double2& b, bool rows32) {
+  swap_dw(a.x, b.x, rows32);
+  swap_dw(a.y, b.y, rows32);
+}
+
+// The exchange between the two LDS passes of the 1024-point per-wave FFT
+// (P = 16, radix 16, then P = 256, radix 4), in registers.  Lane (a, c) =
+// 16 a + c holds in t[4 g + s] the pass-16 output of index 256 a + 64 g +
+// 16 s + c; the P = 256 pass wants in lane (s, c), at t[4 g + a], the element
+// of index (16 s + c) + 64 g + 256 a.  For each g that is the transpose of a
+// 4 x 4 block between the lane row (a) and the register (s), done as a swap
+// of lane-row bit 1 with register bit 1 (pl32 on the pairs (0, 2), (1, 3))
+// and then of bit 0 with bit 0 (pl16 on the pairs (0, 1), (2, 3)).  It takes
+// the place of one LDS store and load of the whole vector (16 KB per wave).
+__device__ __forceinline__ void transpose_1024_p16_p256(double2 (&t)[16]) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    swap_c(t[4 * g + 0], t[4 * g + 2], true);
+    swap_c(t[4 * g + 1], t[4 * g + 3], true);
+    swap_c(t[4 * g + 0], t[4 * g + 1], false);
+    swap_c(t[4 * g + 2], t[4 * g + 3], false);
   }
 }
 
@@ -538,7 +619,21 @@ template <int N, int SIGN, class TW>
 __device__ __forceinline__ void wave_fft(double2 (&v)[16], double2* buf, int lp, const TW& tw) {
   constexpr int E = WaveLay<N>::E;
   dft_any<E, SIGN>(v);                     // pass 1 (P = 1): butterfly lp
-  if constexpr (N > E) {
+  if constexpr (N == 1024 && E == 16) {
+    // one LDS exchange (after the register pass); the second in registers
+#pragma unroll
+    for (int r = 0; r < E; ++r) buf[swz(lp * E + r)] = v[r];
+    wave_sync();
+    pass_load<N, 16>(v, buf, lp);
+    pass_compute<N, 16, SIGN>(v, lp, tw);
+    transpose_1024_p16_p256(v);
+    pass_compute<N, 256, SIGN>(v, lp, tw);
+    double2 t[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) t[r] = v[r];
+    pass_final<N, 256>(t, v);
+    wave_sync();                           // buf may be rewritten by the caller
+  } else if constexpr (N > E) {
 #pragma unroll
     for (int r = 0; r < E; ++r) buf[swz(lp * E + r)] = v[r];
     wave_sync();

@@ -28,12 +28,14 @@ constexpr long kMaxSpec = 512L * 257;
 constexpr int kArrays = 10;                  // new_[3] + mix[7]
 constexpr long kArraySlots = kArrays * kMaxSpec;
 constexpr int kMaxWork = 1024;               // work-list entries (row and column jobs)
-constexpr long kTailBytes = 16384;           // descriptor, work list, W(0,0), flag
-static_assert(((sizeof(RunDesc) + 255) & ~(size_t)255) + sizeof(int2) * kMaxWork + 16 <= (size_t)kTailBytes,
+constexpr int kMaxSide = 1024;               // longest Ny (integrating-factor rows)
+constexpr long kTailBytes = 16384 + 4 * kMaxSide * 8;   // descriptor, work list, W(0,0), flag; IF rows
+static_assert(((sizeof(RunDesc) + 255) & ~(size_t)255) + sizeof(int2) * kMaxWork + 16 + 4 * kMaxSide * 8 <=
+                  (size_t)kTailBytes,
               "scratch tail");
 constexpr long kScratchSlots = kArraySlots + kTailBytes / (long)sizeof(double2);
 
-// 21,069,824 bytes per GPU.
+// 21,102,592 bytes per GPU.
 __device__ __align__(256) double2 scratch[kScratchSlots];
 
 constexpr int kMaxDevices = 64;
@@ -59,13 +61,16 @@ int scratch_ptr(double2** out) {
 }
 
 // Descriptor into the scratch tail, the identity work list (run 0, job k),
-// and the non-finite flag cleared.
-__global__ __launch_bounds__(256) void k_noalloc_setup(RunDesc d, RunDesc* dd, int2* work, int nwork, int* flag) {
+// the integrating-factor rows (those of ifac_rows), and the non-finite flag
+// cleared.
+__global__ __launch_bounds__(256) void k_noalloc_setup(RunDesc d, RunDesc* dd, int2* work, int nwork, int* flag,
+                                                       double* ify, int Ny) {
   if (threadIdx.x == 0) {
     *dd = d;
     *flag = 0;
   }
   for (int k = threadIdx.x; k < nwork; k += blockDim.x) work[k] = make_int2(0, k);
+  for (int j = threadIdx.x; j < Ny; j += blockDim.x) ifac_row(j, Ny, d.dky, d.Re, d.DQ, d.dt, ify);
 }
 
 __global__ void k_noalloc_flag(const int* flag, int* out) { *out = *flag; }
@@ -110,10 +115,12 @@ extern "C" int qs_step_ifrk2_noalloc(const Mesh* mesh, const qs_params* p, const
   int2* work = (int2*)(tail + ((sizeof(RunDesc) + 255) & ~(size_t)255));
   double* w00 = (double*)(work + kMaxWork);
   int* flag = (int*)(w00 + 1);
+  double* ify = w00 + 2;   // [4][Ny]
   d.Ra = Ra;
   d.alpha_p = nullptr;
   d.w00 = d.w00o = w00;
   d.flag = flag;
+  d.ify = ify;
   d.dkx = 2 * M_PI / mesh->Lx;   // the mesh tables' own expressions (Basic/Mesh.cuh:6-21)
   d.dky = 2 * M_PI / mesh->Ly;
   d.alpha = p->alpha; d.lambda = p->lambda;
@@ -131,7 +138,7 @@ extern "C" int qs_step_ifrk2_noalloc(const Mesh* mesh, const qs_params* p, const
   const double2* tw = Nx % 3 == 0 ? mesh->priv->tw768 : mesh->priv->tw1024;
   const hipStream_t s = stream_of(mesh);
 
-  hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(256), 0, s, d, dd, work, nwork, flag);
+  hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(256), 0, s, d, dd, work, nwork, flag, ify, Ny);
   QS_CHECK_LAUNCH();
   // The inverse set of the caller's state (the scratch keeps nothing from an
   // earlier call), then the steps: the same kernels and arithmetic as

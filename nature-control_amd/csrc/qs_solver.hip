@@ -28,6 +28,7 @@ RunDesc make_desc(qs_solver* S) {
   d.Ra = S->Ra;
   d.w00 = d.w00o = S->w00;
   d.flag = S->flag;
+  d.ify = S->ify;
   d.dkx = 2 * M_PI / m->Lx;   // the mesh tables' own expressions (Basic/Mesh.cuh:6-21)
   d.dky = 2 * M_PI / m->Ly;
   d.alpha = S->p.alpha; d.lambda = S->p.lambda;
@@ -213,6 +214,7 @@ int qs_solver_create(Mesh* mesh, const qs_params* params, qs_solver** out) {
   for (int k = 0; k < 7 && ok; ++k) ok = hipMalloc((void**)&S->mix[k], hs) == hipSuccess;
   ok = ok && hipMalloc((void**)&S->w00, sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void**)&S->flag, sizeof(int)) == hipSuccess;
+  ok = ok && hipMalloc((void**)&S->ify, sizeof(double) * 4 * (size_t)mesh->Ny) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&S->hflag, sizeof(int), hipHostMallocDefault) == hipSuccess;
   if (S->hflag) *S->hflag = 0;
   ok = ok && hipMalloc((void**)&S->d_desc, sizeof(RunDesc)) == hipSuccess;
@@ -235,6 +237,8 @@ int qs_solver_create(Mesh* mesh, const qs_params* params, qs_solver** out) {
     st = fail(QS_ERR_HIP, "qs_solver_create: memset");
   if (st == QS_OK && hipMemcpy(S->d_work, work.data(), sizeof(int2) * nwork, hipMemcpyHostToDevice) != hipSuccess)
     st = fail(QS_ERR_HIP, "qs_solver_create: work list");
+  if (st == QS_OK)
+    st = ifac_rows_launch(S->ify, mesh->Ny, 2 * M_PI / mesh->Ly, params->Re, params->D_Q, params->dt, sm);
   if (st == QS_OK) st = upload_desc(S);
   if (st == QS_OK && hipStreamSynchronize(sm) != hipSuccess) st = fail(QS_ERR_HIP, "qs_solver_create: sync");
   if (st != QS_OK) {
@@ -257,7 +261,7 @@ int qs_solver_destroy(qs_solver* S) {
   }
   for (int k = 0; k < 7; ++k)
     if (S->mix[k]) (void)hipFree(S->mix[k]);
-  for (void* p : {(void*)S->Ra, (void*)S->w00, (void*)S->flag, (void*)S->d_desc, (void*)S->d_work})
+  for (void* p : {(void*)S->Ra, (void*)S->w00, (void*)S->flag, (void*)S->d_desc, (void*)S->d_work, (void*)S->ify})
     if (p) (void)hipFree(p);
   if (S->hflag) (void)hipHostFree(S->hflag);
   for (auto* f : S->F) qs_field_destroy(f);
