@@ -77,7 +77,7 @@ def copy_calibrate(mib):
 
 
 # Sources of the fused kernels, in the order their sha256 is taken.
-FUSED_SOURCES = ("qs_fused.hip", "qs_fused_dev.hpp", "qs_fused.hpp", "qs_internal.hpp")
+FUSED_SOURCES = ("qs_fused.hip", "qs_fused_kern.hpp", "qs_fused_dev.hpp", "qs_fused.hpp", "qs_internal.hpp")
 
 
 def traffic_from_profile(kernel, N):

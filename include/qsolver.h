@@ -296,6 +296,9 @@ typedef struct qs_ensemble_config {
   int order;           /* work lists: 0 run-major, 1 job-major (interleaved runs) */
   int row_blocks[8];   /* persistent blocks of the row kernel per size class, 0 = one per job */
   int col_blocks[8];   /* same for the column kernels (classes in ascending size) */
+  int lanes[8];        /* elements per lane of the per-wave FFT per size class (its first-pass radix;
+                          rows per row job and column-tile width follow), 0 = the library default;
+                          sides 64, 128, 256 also take 4 or 8 (256: 4 or 16), 512 also takes 8 */
 } qs_ensemble_config;
 int qs_ensemble_create(int n, const int* N, const qs_params* params, qs_ensemble** out);
 int qs_ensemble_destroy(qs_ensemble* e);
@@ -310,11 +313,12 @@ int qs_ensemble_set_activity(qs_ensemble* e, int run, const Qreal* Ra);
 int qs_ensemble_step(qs_ensemble* e, int nsteps);
 int qs_ensemble_get_config(const qs_ensemble* e, qs_ensemble_config* cfg);
 int qs_ensemble_set_config(qs_ensemble* e, const qs_ensemble_config* cfg);
-/* Autotuner: coordinate descent over branch concurrency, work-list order and
- * per-class block counts, each candidate timed over trial_steps directly
- * launched steps (the graph depth is kept; no graph is captured while tuning);
- * the states are saved and restored.  Sets and returns the fastest
- * configuration and its time per (directly launched) ensemble step in ms. */
+/* Autotuner: coordinate descent over branch concurrency, work-list order,
+ * per-class FFT lane layout (radix / tile width) and block counts, each
+ * candidate timed over trial_steps directly launched steps, then over the
+ * graph depth (0, 5, 10, 20 steps per captured graph, each candidate captured
+ * and timed through graph launches); the states are saved and restored.  Sets
+ * and returns the fastest configuration and its time per ensemble step in ms. */
 int qs_ensemble_tune(qs_ensemble* e, int trial_steps, qs_ensemble_config* best, double* best_ms);
 
 /* Batched discrete adjoint of an ensemble whose runs share one mesh size: the

@@ -28,6 +28,7 @@ struct EnsClass {
   int2* d_row = nullptr;
   int2* d_col = nullptr;
   int nrow = 0, ncol = 0;
+  int cap_row = 0, cap_col = 0;     // entries allocated in d_row, d_col
   hipStream_t side = nullptr;
   hipEvent_t done = nullptr;       // join event of direct launches
   hipEvent_t cap_done = nullptr;   // join event used only inside graph captures
@@ -50,6 +51,7 @@ struct qs_ensemble {
   hipGraphExec_t exec = nullptr;
   int exec_steps = 0;
   int order_built = -1;
+  int lanes_built[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
   qs::AdjGroup* adj = nullptr;             // batched adjoint (one mesh size)
 };
 
@@ -66,11 +68,15 @@ int upload_descs(qs_ensemble* E) {
 
 // Work lists of every class in the configured order.
 int build_work(qs_ensemble* E) {
-  if (E->order_built == E->cfg.order) return QS_OK;
-  for (auto& c : E->cls) {
+  bool same = E->order_built == E->cfg.order;
+  for (size_t ci = 0; ci < E->cls.size(); ++ci) same = same && E->lanes_built[ci] == E->cfg.lanes[ci];
+  if (same) return QS_OK;
+  for (size_t ci = 0; ci < E->cls.size(); ++ci) {
+    EnsClass& c = E->cls[ci];
     std::vector<int2> row, col;
     const RunDesc d0 = make_desc(E->runs[c.members[0]]);
-    const int nr = row_jobs(d0), nc = col_jobs(d0);
+    const int el = E->cfg.lanes[ci];
+    const int nr = row_jobs(d0, el), nc = col_jobs(d0, el);
     if (E->cfg.order == 0) {
       for (int r : c.members) {
         for (int j = 0; j < nr; ++j) row.push_back(make_int2(r, j));
@@ -82,15 +88,23 @@ int build_work(qs_ensemble* E) {
       for (int j = 0; j < nc; ++j)
         for (int r : c.members) col.push_back(make_int2(r, j));
     }
-    if (!c.d_row) {
+    if ((int)row.size() > c.cap_row || (int)col.size() > c.cap_col) {
+      (void)hipStreamSynchronize(E->stream);
+      if (c.d_row) (void)hipFree(c.d_row);
+      if (c.d_col) (void)hipFree(c.d_col);
+      c.d_row = c.d_col = nullptr;
       if (hipMalloc((void**)&c.d_row, sizeof(int2) * row.size()) != hipSuccess ||
           hipMalloc((void**)&c.d_col, sizeof(int2) * col.size()) != hipSuccess)
         return fail(QS_ERR_OOM, "ensemble work lists");
+      c.cap_row = (int)row.size();
+      c.cap_col = (int)col.size();
     }
+    if (E->lanes_built[ci] != el) c.valid = false;   // inverse sets from this layout's prologue
     QS_HIP(hipMemcpy(c.d_row, row.data(), sizeof(int2) * row.size(), hipMemcpyHostToDevice));
     QS_HIP(hipMemcpy(c.d_col, col.data(), sizeof(int2) * col.size(), hipMemcpyHostToDevice));
     c.nrow = (int)row.size();
     c.ncol = (int)col.size();
+    E->lanes_built[ci] = el;
   }
   E->order_built = E->cfg.order;
   return QS_OK;
@@ -103,10 +117,11 @@ int class_step(qs_ensemble* E, int ci, hipStream_t st) {
   EnsClass& c = E->cls[ci];
   const double2* tw = c.mesh->priv->tw1024;
   const int rb = blocks(E->cfg.row_blocks[ci], c.nrow), cb = blocks(E->cfg.col_blocks[ci], c.ncol);
-  QS_TRY(launch_fused_row(c.N, E->d_desc, c.d_row, c.nrow, rb, tw, st));
-  QS_TRY(launch_fused_col(c.N, MODE_S1, E->d_desc, c.d_col, c.ncol, cb, tw, st));
-  QS_TRY(launch_fused_row(c.N, E->d_desc, c.d_row, c.nrow, rb, tw, st));
-  QS_TRY(launch_fused_col(c.N, MODE_S2, E->d_desc, c.d_col, c.ncol, cb, tw, st));
+  const int el = E->cfg.lanes[ci];
+  QS_TRY(launch_fused_row_l(c.N, el, E->d_desc, c.d_row, c.nrow, rb, tw, st));
+  QS_TRY(launch_fused_col_l(c.N, el, MODE_S1, E->d_desc, c.d_col, c.ncol, cb, tw, st));
+  QS_TRY(launch_fused_row_l(c.N, el, E->d_desc, c.d_row, c.nrow, rb, tw, st));
+  QS_TRY(launch_fused_col_l(c.N, el, MODE_S2, E->d_desc, c.d_col, c.ncol, cb, tw, st));
   return QS_OK;
 }
 
@@ -166,8 +181,8 @@ int prologues(qs_ensemble* E) {
   for (size_t ci = 0; ci < E->cls.size(); ++ci) {
     EnsClass& c = E->cls[ci];
     if (c.valid) continue;
-    QS_TRY(launch_fused_col(c.N, MODE_PRO, E->d_desc, c.d_col, c.ncol, blocks(E->cfg.col_blocks[ci], c.ncol),
-                            c.mesh->priv->tw1024, E->stream));
+    QS_TRY(launch_fused_col_l(c.N, E->cfg.lanes[ci], MODE_PRO, E->d_desc, c.d_col, c.ncol,
+                              blocks(E->cfg.col_blocks[ci], c.ncol), c.mesh->priv->tw1024, E->stream));
     c.valid = true;
   }
   return QS_OK;
@@ -329,17 +344,28 @@ int qs_ensemble_set_config(qs_ensemble* E, const qs_ensemble_config* cfg) {
   if (!E || !cfg) return fail(QS_ERR_INVALID_ARG, "null argument");
   if (cfg->graph_steps < 0 || (cfg->order != 0 && cfg->order != 1))
     return fail(QS_ERR_INVALID_ARG, "qs_ensemble_set_config: graph_steps %d order %d", cfg->graph_steps, cfg->order);
+  for (size_t ci = 0; ci < E->cls.size(); ++ci) {
+    if (cfg->lanes[ci] == 0) continue;
+    int el[4];
+    const int n = lane_layouts(E->cls[ci].N, el);
+    bool ok = false;
+    for (int k = 0; k < n; ++k) ok = ok || el[k] == cfg->lanes[ci];
+    if (!ok)
+      return fail(QS_ERR_UNSUPPORTED_SIZE, "qs_ensemble_set_config: %d elements per lane at side %d", cfg->lanes[ci],
+                  E->cls[ci].N);
+  }
   E->cfg = *cfg;
   drop_graph(E);
   return build_work(E);
 }
 
 // Coordinate-descent autotuner over the launch configuration: work-list
-// order, branch concurrency and per-class block counts.  Each candidate is
-// timed (HIP events on the ensemble's stream) over `trial_steps` directly
-// launched steps after one warm-up step; no graph is captured while tuning
-// (the configured graph depth is kept, and the graph is captured at the next
-// qs_ensemble_step).  The states are saved first and restored afterwards.
+// order, branch concurrency, per-class lane layout of the per-wave FFT (its
+// first-pass radix, rows per row job and column-tile width) and block counts,
+// each candidate timed (HIP events on the ensemble's stream) over about
+// `trial_steps` directly launched steps after a warm-up; then the graph depth,
+// each candidate captured and timed through its graph launches.  The states
+// are saved first and restored afterwards.
 int qs_ensemble_tune(qs_ensemble* E, int trial_steps, qs_ensemble_config* best_out, double* best_ms) {
   if (!E) return fail(QS_ERR_INVALID_ARG, "null ensemble");
   if (trial_steps <= 0) return fail(QS_ERR_INVALID_ARG, "trial_steps must be > 0");
@@ -359,21 +385,26 @@ int qs_ensemble_tune(qs_ensemble* E, int trial_steps, qs_ensemble_config* best_o
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (st == QS_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     st = fail(QS_ERR_HIP, "tune: events");
-  const int graph_steps = E->cfg.graph_steps;
+  // One candidate: steps through the candidate's captured graph (G > 0,
+  // captured at its first launch, G steps per launch) or direct launches
+  // (G = 0), after one warm-up launch.  Re-capturing is safe: set_config drops
+  // the previous graph only once no launch of it can still run (drop_graph).
   auto measure = [&](qs_ensemble_config c, double* ms) -> int {
-    c.graph_steps = 0;                                        // direct launches while tuning
     QS_TRY(qs_ensemble_set_config(E, &c));
-    QS_TRY(ensemble_steps(E, 1));                             // warm-up
+    const int G = c.graph_steps;
+    const int n = G > 0 ? std::max(1, (trial_steps + G - 1) / G) * G : trial_steps;
+    QS_TRY(ensemble_steps(E, G > 0 ? G : 1));                 // warm-up (and capture)
     QS_HIP(hipEventRecord(e0, E->stream));
-    QS_TRY(ensemble_steps(E, trial_steps));
+    QS_TRY(ensemble_steps(E, n));
     QS_HIP(hipEventRecord(e1, E->stream));
     QS_HIP(hipEventSynchronize(e1));
     float t = 0.f;
     QS_HIP(hipEventElapsedTime(&t, e0, e1));
-    *ms = t / trial_steps;
+    *ms = t / n;
     return QS_OK;
   };
   qs_ensemble_config best = E->cfg;
+  best.graph_steps = 0;                                       // launch shapes first, directly launched
   double bms = 0.0;
   if (st == QS_OK) st = measure(best, &bms);
   const int nc = (int)E->cls.size();
@@ -393,15 +424,26 @@ int qs_ensemble_tune(qs_ensemble* E, int trial_steps, qs_ensemble_config* best_o
       qs_ensemble_config c = best; c.order = v;
       if (st == QS_OK) st = try_cfg(c);
     }
-    for (int ci = 0; ci < nc && st == QS_OK; ++ci)
+    for (int ci = 0; ci < nc && st == QS_OK; ++ci) {
+      int el[4];
+      const int nl = lane_layouts(E->cls[ci].N, el);
+      for (int k = 0; k < nl && st == QS_OK; ++k) {
+        qs_ensemble_config c = best; c.lanes[ci] = el[k];
+        st = try_cfg(c);
+      }
       for (int b : {0, 128, 256, 512, 1024, 2048}) {
         qs_ensemble_config c = best; c.row_blocks[ci] = b;
         if (st == QS_OK) st = try_cfg(c);
         c = best; c.col_blocks[ci] = b;
         if (st == QS_OK) st = try_cfg(c);
       }
+    }
   }
-  best.graph_steps = graph_steps;
+  // graph depth, each candidate captured and launched as a graph
+  for (int G : {5, 10, 20}) {
+    qs_ensemble_config c = best; c.graph_steps = G;
+    if (st == QS_OK) st = try_cfg(c);
+  }
   if (st == QS_OK) st = qs_ensemble_set_config(E, &best);
   // restore states
   for (int r = 0; r < n; ++r) {

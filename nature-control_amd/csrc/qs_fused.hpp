@@ -49,9 +49,10 @@ struct RunDesc {
   double2* scratch;               // row adjoint: [grid][6][1024] physical product adjoints
 };
 
-// Jobs of one run per launch (row: WaveLay<Nx>::NV rows each; column: 3 fields x tiles).
-int row_jobs(const RunDesc& a);
-int col_jobs(const RunDesc& a);
+// Jobs of one run per launch (row: WaveLay<Nx, el>::NV rows each; column: 3
+// fields x tiles of 4 NV columns); el = 0: the default layout wave_e.
+int row_jobs(const RunDesc& a, int el = 0);
+int col_jobs(const RunDesc& a, int el = 0);
 
 // work[k] = (run index, job index).  grid <= nwork; blocks loop over the list.
 // nfix: the size common to all listed runs in the kernel's FFT direction (Nx
@@ -61,6 +62,18 @@ int launch_fused_row(int nfix, const RunDesc* runs, const int2* work, int nwork,
                      const double2* tw1024, hipStream_t st);
 int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, int nwork, int grid,
                      const double2* tw1024, hipStream_t st);
+
+// The same kernels with another lane layout of the per-wave FFT: el elements
+// per lane (0 or wave_e(nfix): the default).  Other layouts exist for the
+// sides 64, 128, 256 and 512 (lane_layouts) and the modes of the IFRK2 step
+// (MODE_PRO, MODE_S1, MODE_S2); anything else is QS_ERR_UNSUPPORTED_SIZE.
+int launch_fused_row_l(int nfix, int el, const RunDesc* runs, const int2* work, int nwork, int grid,
+                       const double2* tw1024, hipStream_t st);
+int launch_fused_col_l(int nfix, int el, int mode, const RunDesc* runs, const int2* work, int nwork, int grid,
+                       const double2* tw1024, hipStream_t st);
+// Layouts (elements per lane) available for side n, the default first; returns
+// their count (at most 4).
+int lane_layouts(int n, int* el);
 
 // Discrete adjoint kernels (qs_adjoint.hip).  nfix > 0 only.
 int launch_adj_row(int nfix, const RunDesc* runs, const int2* work, int nwork, int grid,

@@ -137,9 +137,9 @@ struct IFac {
 // H = N/2+1 bins each) into sm[t*RH + vec*H + k].  RH = NV*H <= 576 < 2*384:
 // two slots per thread and array, all 14 loads in flight before the first LDS
 // write.  Ends with a barrier.
-template <int N>
+template <int N, int EL = wave_e(N)>
 __device__ __forceinline__ void stage_rows(double2* const* arr, const RunDesc& a, int row0, double2* sm) {
-  constexpr int NV = WaveLay<N>::NV, H = N / 2 + 1, RH = NV * H;
+  constexpr int NV = WaveLay<N, EL>::NV, H = N / 2 + 1, RH = NV * H;
   static_assert(RH <= 2 * 384, "row staging assumes two slots per thread");
   const int tid = threadIdx.x;
   double2 ld[7][2];
@@ -173,9 +173,9 @@ __device__ __forceinline__ void stage_rows(double2* const* arr, const RunDesc& a
 // released the wave buffers.  Only these waves wait on the loads; a compute
 // wave's wait would also drain its own pending stores (loads and stores share
 // one counter).  S = ceil(RH / 128) slots per array and thread.
-template <int N>
+template <int N, int EL = wave_e(N)>
 struct RowLoader {
-  static constexpr int NV = WaveLay<N>::NV, H = N / 2 + 1, RH = NV * H, S = (RH + 127) / 128;
+  static constexpr int NV = WaveLay<N, EL>::NV, H = N / 2 + 1, RH = NV * H, S = (RH + 127) / 128;
   double2 ld[7][S];
   double w00, alu;   // the run's W(0,0) and the step's uniform activity
   // rows row0 .. row0+NV-1 of a's inverse set (out-of-range slots read a
@@ -219,10 +219,10 @@ struct RowLoader {
 // (vector g of wave buffer q at sm[q*kBuf + g*VS + swz(x)]).  v: scratch
 // registers of the caller; w00: the run's W(0,0) (*a.w00).  Ends with a
 // barrier.
-template <int N, class TW>
+template <int N, int EL = wave_e(N), class TW>
 __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, double2* sm, const TW& tw,
                                                double2 (&v)[16], double w00) {
-  using L = WaveLay<N>;
+  using L = WaveLay<N, EL>;
   constexpr int NV = L::NV, LPC = L::LPC, VS = L::VS, E = L::E, H = N / 2 + 1, RH = NV * H;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -264,7 +264,7 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
   __syncthreads();   // staging area free
   if (wave < 5) {
     const double sc = a.scale;
-    wave_fft<N, +1>(v, vb, lp, tw);
+    wave_fft<N, +1, EL>(v, vb, lp, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) vb[swz(lp + LPC * m)] = make_double2(v[m].x * sc, v[m].y * sc);
   }
@@ -277,10 +277,10 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
 // covers whole row segments of the tile, while the FFTs run per wavefront on
 // the lane layout "lane (lp, g) holds rows lp + LPC*m, m < E".  A tile holds
 // TC * NY = 256 * E elements: E per thread.
-template <int NY>
+template <int NY, int EL = wave_e(NY)>
 struct ColTile {
-  static constexpr int CW = WaveLay<NY>::NV, TC = 4 * CW, LPC = WaveLay<NY>::LPC, VS = WaveLay<NY>::VS;
-  static constexpr int E = WaveLay<NY>::E;
+  static constexpr int CW = WaveLay<NY, EL>::NV, TC = 4 * CW, LPC = WaveLay<NY, EL>::LPC, VS = WaveLay<NY, EL>::VS;
+  static constexpr int E = WaveLay<NY, EL>::E;
   static constexpr int WS = kBuf + 1;   // odd wave stride: spreads LDS banks
   const RunDesc& a;
   double2* sm;
@@ -376,9 +376,9 @@ __device__ __forceinline__ void col_job_map(int job, int ntiles, int& fld, int& 
 // 64 entries (w*E + q)*64 + lane: whole row segments, as ColTile::load reads
 // them.  Columns past the last one (i >= Nxh) repeat it; their results are
 // never stored.
-template <int NY>
+template <int NY, int EL = wave_e(NY)>
 struct TileDMA {
-  static constexpr int CW = WaveLay<NY>::NV, TC = 4 * CW, LPC = WaveLay<NY>::LPC, E = WaveLay<NY>::E;
+  static constexpr int CW = WaveLay<NY, EL>::NV, TC = 4 * CW, LPC = WaveLay<NY, EL>::LPC, E = WaveLay<NY, EL>::E;
   double2* D;
   const RunDesc& a;
   int i0;

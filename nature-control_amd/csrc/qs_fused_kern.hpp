@@ -1,0 +1,461 @@
+// qs_fused_kern.hpp -- the fused right-hand-side kernels (row job, column job,
+// their kernels and launchers), templated on the mesh side and on the lane
+// layout of the per-wave FFT (elements per lane EL, default wave_e(N)).
+// Included by qs_fused.hip (the default layouts, every mode and size) and by
+// qs_fused_alt.hip (the other layouts the ensemble autotuner may choose).
+#pragma once
+
+#include "qs_fused_dev.hpp"
+
+namespace qs {
+namespace {
+
+// Row job layout (one job = NV = WaveLay<N>::NV rows, 6 compute waves, 6 LDS
+// buffers):
+//  1. the 7 mixed rows are loaded cooperatively (coalesced) into a staging
+//     view of the buffers (stage_rows, or the loader waves' RowLoader);
+//  2. waves 0-4 build their two-for-one C2R inputs from the staging area,
+//     pairing fields of like magnitude: (w, dy u + dx v), (u, v), (r1, r2),
+//     (L1, L2), (dx u, -);
+//  3. inverse x FFTs (per wave) -> physical fields in buffers 0-4;
+//  4. pointwise products (all threads) -> buffers 0-5 as R2C pairs
+//     (A, B), (-u r1, -v r1), (-u r2, -v r2), (P11, P12), (P21, -), (loc1, loc2);
+//  5. forward x FFTs (per wave), spectra back into the buffers;
+//  6. cooperative unpack + spectral combination -> F0..F6 (coalesced stores).
+// The part of a row job after staging: phases 3-6, run by threads 0..383.
+// w00 = *a.w00; alu: the step's uniform activity (*a.alpha_p, or a.alpha).
+// Block barriers: kRowBodyBarriers (two in inverse_fields, one after the
+// products, one after the forward FFTs).
+constexpr int kRowBodyBarriers = 4;
+template <int N, int EL = wave_e(N), class TW>
+__device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm, const TW& tw, double w00,
+                                         double alu) {
+  using L = WaveLay<N, EL>;
+  constexpr int NV = L::NV;             // rows of this job (= vectors per wave FFT)
+  constexpr int LPC = L::LPC, E = L::E;
+  constexpr int VS = L::VS;             // padded vector stride inside a wave buffer
+  constexpr int H = N / 2 + 1;
+  constexpr int RH = NV * H;            // staged bins per array
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int lp = lane / NV, g = lane % NV;     // lane = lp * NV + g
+  double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
+
+  double2 v[16];
+  inverse_fields<N, EL>(a, row0, sm, tw, v, w00);
+
+  // ---- 4. pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) --
+  for (int p = tid; p < NV * N; p += 384) {
+    const int gg = p / N, x = p % N;
+    const int o = gg * VS + swz(x);
+    const int yy = row0 + gg;
+    const double2 f0 = sm[0 * kBuf + o], f1 = sm[1 * kBuf + o], f2 = sm[2 * kBuf + o],
+                  f3 = sm[3 * kBuf + o], f4 = sm[4 * kBuf + o];
+    const double w = f0.x, uyvx = f0.y, u = f1.x, v = f1.y, r1 = f2.x, r2 = f2.y;
+    const double L1 = f3.x, L2 = f3.y, ux = f4.x;
+    const double al = a.Ra ? gload(a.Ra + (size_t)(yy < a.Ny ? yy : a.Ny - 1) * N + x) : alu;
+    const double S = 2 * sqrt(r1 * r1 + r2 * r2);
+    const double cS2 = a.cQ * S * S;
+    const double gg1 = a.lambda * S * (cS2 - a.aQ);
+    const double mlD = -1.0 * a.lambda * a.DQ;
+    const double s1 = mlD * L1 * S + gg1 * r1;                          // single(r1)
+    const double s2 = mlD * L2 * S + gg1 * r2;                          // single(r2)
+    const double cr = 2.0 * a.DQ * L1 * r2 + (-2.0 * a.DQ) * r1 * L2;   // cross(r1, r2)
+    const double p11 = s1 - al * r1;
+    const double p12 = (s2 + cr) - al * r2;
+    const double p21 = (s2 - cr) - al * r2;
+    const double pA = a.advect_w ? -1.0 * u * w : 0.0;
+    const double pB = a.advect_w ? -1.0 * v * w : 0.0;
+    const double loc1 = (-1.0 * w * r2 - cS2 * r1) + a.lambda * S * ux;
+    const double loc2 = (1.0 * w * r1 - cS2 * r2) + 0.5 * a.lambda * S * uyvx;
+    sm[0 * kBuf + o] = make_double2(pA, pB);
+    sm[1 * kBuf + o] = make_double2(-1.0 * u * r1, -1.0 * v * r1);
+    sm[2 * kBuf + o] = make_double2(-1.0 * u * r2, -1.0 * v * r2);
+    sm[3 * kBuf + o] = make_double2(p11, p12);
+    sm[4 * kBuf + o] = make_double2(p21, 0.0);
+    sm[5 * kBuf + o] = make_double2(loc1, loc2);
+  }
+  __syncthreads();
+
+  // ---- 5. forward x FFTs ------------------------------------------------------
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = vb[swz(lp + LPC * m)];
+  wave_sync();
+  wave_fft<N, -1, EL>(v, vb, lp, tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) vb[swz(lp + LPC * m)] = v[m];
+  __syncthreads();
+
+  // ---- 6. unpack the two-for-one spectra and form F0..F6 ---------------------
+  const double sE = a.s_ErRe;
+  for (int e = tid; e < RH; e += 384) {
+    const int vec = e / H, k = e % H;
+    const int y = row0 + vec;
+    if (y >= a.Ny) continue;
+    constexpr bool POW2 = (N & (N - 1)) == 0;
+    const int o = vec * VS + swz(k), oc = vec * VS + swz(POW2 ? ((N - k) & (N - 1)) : (k == 0 ? 0 : N - k));
+    double2 Sp[6][2];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const double2 zk = sm[q * kBuf + o], zc = cconj(sm[q * kBuf + oc]);
+      Sp[q][0] = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+      const double2 d = csub(zk, zc);
+      Sp[q][1] = make_double2(0.5 * d.y, -0.5 * d.x);
+    }
+    const double2 A = Sp[0][0], B = Sp[0][1], U1 = Sp[1][0], V1 = Sp[1][1], U2 = Sp[2][0], V2 = Sp[2][1];
+    const double2 P11 = Sp[3][0], P12 = Sp[3][1], P21 = Sp[4][0], Lc1 = Sp[5][0], Lc2 = Sp[5][1];
+    const double kx = a.dkx * k;
+    const double kxa = (2 * k == N) ? 0.0 : kx;      // projected x-derivative of the advection
+    const double c12 = -kx * kx * sE, t11 = 2.0 * kx * sE;
+    const size_t ro = (size_t)y * a.pitch + k;
+    gstore(a.mix[0] + ro, make_double2(-kxa * A.y + c12 * P12.x, kxa * A.x + c12 * P12.y));  // i kx A - kx^2 s P12
+    gstore(a.mix[1] + ro, make_double2(B.x + t11 * P11.y, B.y - t11 * P11.x));             // B - 2 i kx s P11
+    gstore(a.mix[2] + ro, make_double2(sE * P21.x, sE * P21.y));                             // s P21
+    gstore(a.mix[3] + ro, make_double2(Lc1.x - kx * U1.y, Lc1.y + kx * U1.x));               // loc1 + i kx U1
+    gstore(a.mix[4] + ro, V1);
+    gstore(a.mix[5] + ro, make_double2(Lc2.x - kx * U2.y, Lc2.y + kx * U2.x));               // loc2 + i kx U2
+    gstore(a.mix[6] + ro, V2);
+  }
+}
+
+
+// ===========================================================================
+// Column job
+// ===========================================================================
+// Column job: one field (FLD 0 = w, 1 = r1, 2 = r2) on a tile of 4 NV
+// columns (ColTile), one column per lane group, FFTs along y per wavefront.
+template <int NY, int MODE, int EL = wave_e(NY), class TW>
+__device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const TW& tw,
+                                        double2* dbuf, bool pro_arg) {
+  // prologue: final-stage kernels only (see is_prologue)
+  const bool pro = (MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && pro_arg;
+  constexpr int LPC = WaveLay<NY, EL>::LPC, E = WaveLay<NY, EL>::E;
+  ColTile<NY, EL> T(a, sm, i0);
+  const int lp = T.lp;
+  const int i = i0 + (threadIdx.x >> 6) * ColTile<NY, EL>::CW + T.g;
+  const bool iok = i < a.Nxh;
+  const int ic = iok ? i : 0;
+  double2* vb = T.vb;
+  const double kx = a.dkx * ic;
+  // ky(j) of the mesh tables (Basic/Mesh.cuh:6-21)
+  auto kyj = [&](int j) { return a.dky * (j < NY / 2 + 1 ? j : j - NY); };
+  const int nyq_x = a.Nx / 2;
+  bool bad = false;
+  auto stage_out = [&](double2* __restrict__ dst, const double2 (&v)[16]) { T.stage_out(dst, v); };
+  auto tile_store = [&](double2* __restrict__ dst) { T.store(dst); };
+
+  double2 st[16];   // the state of the next stage (curr / new / old)
+  if (pro) {
+    // straight from HBM in the lane layout (a prologue runs once per restart:
+    // no staging registers, which the stage path needs for its own tiles)
+    const double2* src = a.old_[FLD] + ic;
+#pragma unroll
+    for (int m = 0; m < E; ++m) st[m] = gload(src + (unsigned)((lp + LPC * m) * a.pitch));
+  } else {
+    // ---- N_X from the forward set ------------------------------------------
+    //   N_w = F0 + i ky F1 + ky^2 F2,  N_r = F_a + i ky F_b   (in st)
+    // Input tiles arrive by LDS DMA into dbuf, each issued as soon as the
+    // previous one has been read, so it loads while that one is transformed:
+    // F_a, F_b, (F2,) then the state tile of the stage update.
+    const TileDMA<NY, EL> D{dbuf, a, i0};
+    const int cl = (threadIdx.x >> 6) * ColTile<NY, EL>::CW + T.g;   // this lane's tile column
+    const bool from_old = MODE == MODE_S1 || MODE == MODE_R1 || MODE == MODE_E;
+    const double2* state_in = from_old ? a.old_[FLD] : a.new_[FLD];   // R2, R3: new first
+    const int f0 = FLD == 0 ? 0 : (FLD == 1 ? 3 : 5);
+    D.issue(a.mix[f0]);
+    D.wait();
+    D.read(st, cl, lp);
+    __syncthreads();
+    D.issue(a.mix[f0 + 1]);
+    wave_fft<NY, -1, EL>(st, vb, lp, tw);
+    {
+      double2 v[16];
+      D.wait();
+      D.read(v, cl, lp);
+      __syncthreads();
+      D.issue(FLD == 0 ? a.mix[2] : state_in);
+      wave_fft<NY, -1, EL>(v, vb, lp, tw);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double ky = kyj(lp + LPC * m);
+        st[m] = make_double2(st[m].x - ky * v[m].y, st[m].y + ky * v[m].x);
+      }
+    }
+    if (FLD == 0) {
+      double2 v[16];
+      D.wait();
+      D.read(v, cl, lp);
+      __syncthreads();
+      D.issue(state_in);
+      wave_fft<NY, -1, EL>(v, vb, lp, tw);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double ky = kyj(lp + LPC * m);
+        st[m] = make_double2(st[m].x + ky * ky * v[m].x, st[m].y + ky * ky * v[m].y);
+      }
+    }
+    // ---- stage update (state tiles read and rewritten in LDS) ---------------
+    // N_X is in st.  Per element: IF = exp(L dt), IFh = exp(L dt / 2) (IFac).
+    const IFac ifc(a, FLD, kx, NY);
+    auto keep = [&](int j) {   // dealias mask, Qsolver.cuh:314-322
+      return (i < ((a.Nxh - 1) * 2 / 3 + 1)) && ((j < NY / 3 + 1) || ((NY - j) < NY / 3));
+    };
+#pragma unroll
+    for (int m = 0; m < E; ++m) {   // real part of N_w at the self-conjugate bins
+      const int j = lp + LPC * m;
+      if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
+    }
+    D.wait();   // the state tile is in dbuf; updated values go to the wave buffers for the store
+    if (MODE == MODE_S1 || MODE == MODE_S2 || MODE == MODE_R1 || MODE == MODE_R4 || MODE == MODE_E) {
+      // one state tile: S1 / R1 / E read old, S2 / R4 read new
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int j = lp + LPC * m;
+        double2* sl = vb + swz(j);
+        const double2 in = D.at(j, cl);
+        const double2 n = st[m];
+        const double2 an = make_double2(n.x * a.dt, n.y * a.dt);
+        if (MODE == MODE_S1) {
+          const double IF = ifc.IF(j);
+          const double2 old = in;
+          const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // IFRK2.cuh:16-26
+          const double h = 1.0 / 2.0 * IF;
+          *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // IFRK2.cuh:28-39
+          st[m] = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);      // curr (IFh = IF)
+        } else if (MODE == MODE_R1) {
+          const double IF = ifc.IF(j), IFh = ifc.IFh(j);
+          const double2 old = in;
+          const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // RK4.cuh:12-22
+          const double h = 1.0 / 6.0 * IF;
+          *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // RK4.cuh:25-36
+          st[m] = make_double2((old.x + an.x / 2.0) * IFh, (old.y + an.y / 2.0) * IFh);
+        } else if (MODE == MODE_E) {
+          const double IF = ifc.IF(j);
+          const double2 old = in;                                                // IFE.cuh:15-25
+          double2 r = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);
+          const double cut = keep(j) ? 1.0 : 0.0;
+          r = make_double2(r.x * cut, r.y * cut);
+          if (iok) bad |= !isfinite(r.x) || !isfinite(r.y);
+          *sl = r;
+          st[m] = r;
+        } else {
+          const double2 nw = in;
+          double2 r = MODE == MODE_S2
+                          ? make_double2(nw.x + 1.0 / 2.0 * an.x, nw.y + 1.0 / 2.0 * an.y)   // IFRK2.cuh:41-51
+                          : make_double2(nw.x + 1.0 / 6.0 * an.x, nw.y + 1.0 / 6.0 * an.y);  // RK4.cuh:63-74
+          const double cut = keep(j) ? 1.0 : 0.0;
+          r = make_double2(r.x * cut, r.y * cut);
+          if (iok) bad |= !isfinite(r.x) || !isfinite(r.y);
+          *sl = r;
+          st[m] = r;
+        }
+      }
+      // S1, R1 -> new; S2, R4, E -> the next state over old
+      tile_store((MODE == MODE_S1 || MODE == MODE_R1) ? a.new_[FLD] : a.old_[FLD]);
+    } else {
+      // R2, R3: new += 1/3 IFh dt N (new tile), then curr from old (old tile)
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int j = lp + LPC * m;
+        double2* sl = vb + swz(j);
+        const double2 nw = D.at(j, cl);
+        const double IFh = ifc.IFh(j);
+        const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
+        const double h = 1.0 / 3.0 * IFh;                                          // RK4.cuh:37-61
+        *sl = make_double2(nw.x + h * bn.x, nw.y + h * bn.y);
+      }
+      tile_store(a.new_[FLD]);   // its barriers: every wave has read dbuf
+      D.issue(a.old_[FLD]);
+      D.wait();
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int j = lp + LPC * m;
+        const double2 old = D.at(j, cl);
+        const double IFh = ifc.IFh(j);
+        const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
+        if (MODE == MODE_R2) {
+          st[m] = make_double2(old.x * IFh + bn.x / 2.0, old.y * IFh + bn.y / 2.0);
+        } else {
+          const double IF = ifc.IF(j);
+          st[m] = make_double2(old.x * IF + bn.x * IFh, old.y * IF + bn.y * IFh);
+        }
+      }
+    }
+  }
+
+  // ---- inverse set of the next stage from st ----------------------------------
+  if (FLD == 0) {
+    // M0 = W, M1 = psi = W/k^2, M2 = i ky psi   (vel_func, QFldfuncs.cuh:204-224)
+    if (i == 0 && lp == 0 && a.w00o) *(QS_GLOBAL double*)a.w00o = st[0].x;   // element (j=0, i=0)
+    {
+      double2 v[16];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = st[m];
+      wave_fft<NY, +1, EL>(v, vb, lp, tw);
+      stage_out(a.mout[0], v);
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {   // st <- psi
+      const int j = lp + LPC * m;
+      const double ky = kyj(j);
+      const double k2 = kx * kx + ky * ky;
+      st[m] = (i == 0 && j == 0) || !iok ? make_double2(0.0, 0.0) : make_double2(st[m].x / k2, st[m].y / k2);
+    }
+    {
+      double2 v[16];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = st[m];
+      wave_fft<NY, +1, EL>(v, vb, lp, tw);
+      stage_out(a.mout[1], v);
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {   // st <- i ky psi
+      const double ky = kyj(lp + LPC * m);
+      st[m] = make_double2(-ky * st[m].y, ky * st[m].x);
+    }
+    wave_fft<NY, +1, EL>(st, vb, lp, tw);
+    stage_out(a.mout[2], st);
+  } else {
+    const int m0 = FLD == 1 ? 3 : 5;
+    {
+      double2 v[16];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = st[m];
+      wave_fft<NY, +1, EL>(v, vb, lp, tw);
+      stage_out(a.mout[m0], v);
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {   // st <- -ky^2 R
+      const double ky = kyj(lp + LPC * m);
+      st[m] = make_double2(-ky * ky * st[m].x, -ky * ky * st[m].y);
+    }
+    wave_fft<NY, +1, EL>(st, vb, lp, tw);
+    stage_out(a.mout[m0 + 1], st);
+  }
+  if ((MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && bad) atomicOr(a.flag, 1);
+}
+
+
+// ===========================================================================
+// Kernels: one run (grid over jobs) and an ensemble (work list over runs).
+// ===========================================================================
+
+// Every run of a row kernel's work list has Nx = NFIX (its x FFTs; the row
+// count is a runtime value): six compute waves and two loader waves
+// (RowLoader) per block.
+constexpr int kRowThreads = 512;
+
+template <int NFIX, int EL = wave_e(NFIX)>
+__global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
+                                                    int nwork, const double2* __restrict__ tw1024) {
+  static_assert(NFIX > 0, "one instantiation per x size");
+  constexpr int NT = kRowThreads;
+  __shared__ double2 sm[6 * kBuf];
+  __shared__ double2 twq[kTwq];
+  __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
+  __shared__ double rsc[2];                  // the job's (w00, alu), from the loader
+  stage_twiddles(twq, tw1024, threadIdx.x, NT);
+  __syncthreads();
+  if constexpr (use_twt<NFIX>()) {
+    build_twtab<NFIX>(twt, twq, threadIdx.x, NT);
+    __syncthreads();
+  }
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  constexpr int NV = WaveLay<NFIX, EL>::NV;
+  if (threadIdx.x >= 384) {
+    // Loader waves.  Per job they pass exactly the compute waves' block
+    // barriers: staging ready, the kRowBodyBarriers of row_body, job done.
+    const int lt = threadIdx.x - 384;
+    RowLoader<NFIX, EL> rl;
+    if (L < nwork) {
+      const int2 it = work[L];
+      rl.issue(runs[it.x], it.y * NV, lt);
+    }
+    for (int w = L; w < nwork; w += gridDim.x) {
+      rl.commit(sm, rsc, lt);            // this job's rows (waits for the loads)
+      const int wn = w + gridDim.x;
+      if (wn < nwork) {
+        const int2 it = work[wn];
+        rl.issue(runs[it.x], it.y * NV, lt);   // the next job's, in flight during this one
+      }
+      __syncthreads();   // staging ready
+#pragma unroll 1
+      for (int b = 0; b < kRowBodyBarriers; ++b) __syncthreads();
+      __syncthreads();   // job done
+    }
+  } else {
+    for (int w = L; w < nwork; w += gridDim.x) {
+      const int2 it = work[w];          // (run, job)
+      __syncthreads();   // staging ready
+      row_body<NFIX, EL>(runs[it.x], it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1]);
+      __syncthreads();   // job done: wave buffers free for the next staging
+    }
+  }
+}
+
+template <int NY, int MODE, int EL = wave_e(NY), class TW>
+__device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const TW& tw,
+                                               double2* dbuf, bool pro) {
+  constexpr int tc = 4 * WaveLay<NY, EL>::NV;
+  const int ntiles = (a.Nxh + tc - 1) / tc;
+  int fld, tile;
+  col_job_map(job, ntiles, fld, tile);
+  col_job<NY, MODE, EL>(a, fld, tile * tc, sm, tw, dbuf, pro);
+}
+
+// Every run of a column kernel's work list has Ny = NFIX (its y FFTs; the
+// column count is a runtime value).  pro != 0: the prologue of the
+// final-stage modes (S2, R4, E).
+template <int NFIX, int MODE, int EL = wave_e(NFIX)>
+__global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
+                                             int nwork, const double2* __restrict__ tw1024, int pro) {
+  __shared__ double2 sm[4 * (kBuf + 1)];
+  __shared__ double2 twq[kTwq];
+  __shared__ double2 dbuf[4096];   // DMA tile buffer (TileDMA), 256 * E entries used
+  __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
+  stage_twiddles(twq, tw1024, threadIdx.x, 256);
+  __syncthreads();
+  if constexpr (use_twt<NFIX>()) {
+    build_twtab<NFIX>(twt, twq, threadIdx.x, 256);
+    __syncthreads();
+  }
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  for (int w = L; w < nwork; w += gridDim.x) {
+    const int2 it = work[w];
+    col_dispatch_n<NFIX, MODE, EL>(runs[it.x], it.y, sm, fused_tw<NFIX>(twq, twt), dbuf, pro != 0);
+    __syncthreads();
+  }
+}
+
+template <int NFIX, int EL = wave_e(NFIX)>
+int launch_row_n(const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw, hipStream_t st) {
+  hipLaunchKernelGGL((k_row<NFIX, EL>), dim3(grid), dim3(kRowThreads), 0, st, runs, work, nwork, tw);
+  QS_CHECK_LAUNCH();
+  return QS_OK;
+}
+
+template <int NFIX>
+int launch_col_n(int mode, const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw,
+                 hipStream_t st) {
+  switch (mode) {
+#define QS_COL_MODE(M, K, P) \
+    case M: hipLaunchKernelGGL((k_col<NFIX, K>), dim3(grid), dim3(256), 0, st, runs, work, nwork, tw, P); break;
+    QS_COL_MODE(MODE_PRO, MODE_S2, 1)
+    QS_COL_MODE(MODE_PRO_R4, MODE_R4, 1)
+    QS_COL_MODE(MODE_PRO_E, MODE_E, 1)
+    QS_COL_MODE(MODE_S1, MODE_S1, 0)
+    QS_COL_MODE(MODE_S2, MODE_S2, 0)
+    QS_COL_MODE(MODE_R1, MODE_R1, 0)
+    QS_COL_MODE(MODE_R2, MODE_R2, 0)
+    QS_COL_MODE(MODE_R3, MODE_R3, 0)
+    QS_COL_MODE(MODE_R4, MODE_R4, 0)
+    QS_COL_MODE(MODE_E, MODE_E, 0)
+#undef QS_COL_MODE
+    default: return fail(QS_ERR_INVALID_ARG, "column mode %d", mode);
+  }
+  QS_CHECK_LAUNCH();
+  return QS_OK;
+}
+
+}  // namespace
+}  // namespace qs

@@ -420,17 +420,19 @@ __device__ __forceinline__ void dft12(double2* v) {
 // that already fill the device (4 per lane: 2.25x one run, but ensembles
 // 18 % slower).
 constexpr int wave_e(int N) { return N % 3 == 0 ? 12 : (N == 256 ? 8 : 16); }
-template <int N>
+// EL: elements per lane of a non-default layout (powers of two 4..16 with
+// N/EL <= 64), chosen per size class by the ensemble autotuner.
+template <int N, int EL = wave_e(N)>
 struct WaveLay {
   static constexpr bool TRI = N % 3 == 0;
-  static constexpr int E = wave_e(N);
+  static constexpr int E = EL;
   static constexpr int LPC = N / E;
   static constexpr int NV = 64 / LPC;
   static constexpr int VS = N + N / 16;
   static constexpr int RMAX = TRI ? 4 : (E < 16 ? E : 16);   // radix of the LDS passes
   static_assert(LPC * NV == 64 && LPC * E == N, "one wave holds NV whole vectors");
 };
-constexpr int wave_nv(int N) { return 64 * wave_e(N) / N; }
+constexpr int wave_nv(int N, int el = 0) { return 64 * (el > 0 ? el : wave_e(N)) / N; }
 
 template <int R, int SIGN>
 __device__ __forceinline__ void dft_any(double2* v) {
@@ -460,9 +462,9 @@ __device__ __forceinline__ int swz(int e) { return e + (e >> 4); }
 // inputs from `buf`, twiddles and radix-R butterflies in registers, outputs
 // to `buf` (or into v after the last pass).  t holds butterfly q's inputs at
 // t[q R + r].
-template <int N, int P>
+template <int N, int P, int EL = wave_e(N)>
 struct PassShape {
-  using L = WaveLay<N>;
+  using L = WaveLay<N, EL>;
   static constexpr int R = (N / P >= L::RMAX) ? L::RMAX : N / P;
   static constexpr int Q = N / R;
   static constexpr int BPL = Q / L::LPC;   // butterflies per lane
@@ -471,24 +473,24 @@ struct PassShape {
   static_assert(BPL * R == L::E && N % (P * R) == 0, "pass shape");
 };
 
-template <int N, int P>
+template <int N, int P, int EL = wave_e(N)>
 __device__ __forceinline__ void pass_load(double2 (&t)[16], const double2* buf, int lp) {
-  using S = PassShape<N, P>;
+  using S = PassShape<N, P, EL>;
 #pragma unroll
   for (int q = 0; q < S::BPL; ++q) {
-    const int b = lp + WaveLay<N>::LPC * q;
+    const int b = lp + WaveLay<N, EL>::LPC * q;
 #pragma unroll
     for (int r = 0; r < S::R; ++r) t[q * S::R + r] = buf[swz(b + r * S::Q)];
   }
 }
 
-template <int N, int P, int SIGN, class TW>
+template <int N, int P, int SIGN, int EL = wave_e(N), class TW>
 __device__ __forceinline__ void pass_compute(double2 (&t)[16], int lp, const TW& tw) {
-  using S = PassShape<N, P>;
+  using S = PassShape<N, P, EL>;
   constexpr int R = S::R, STEP = S::STEP;
 #pragma unroll
   for (int q = 0; q < S::BPL; ++q) {
-    const int b = lp + WaveLay<N>::LPC * q;
+    const int b = lp + WaveLay<N, EL>::LPC * q;
     const int k = (int)((unsigned)b % (unsigned)P);   // b >= 0: an AND for P = 2^a
     double2* tq = t + q * R;
     // Twiddles w^r, w = W^(k STEP).  With a per-pass table (TwT): one LDS read
@@ -528,21 +530,21 @@ __device__ __forceinline__ void pass_compute(double2 (&t)[16], int lp, const TW&
   }
 }
 
-template <int N, int P>
+template <int N, int P, int EL = wave_e(N)>
 __device__ __forceinline__ void pass_store(const double2 (&t)[16], double2* buf, int lp) {
-  using S = PassShape<N, P>;
+  using S = PassShape<N, P, EL>;
 #pragma unroll
   for (int q = 0; q < S::BPL; ++q) {
-    const int b = lp + WaveLay<N>::LPC * q;
+    const int b = lp + WaveLay<N, EL>::LPC * q;
     const int k = (int)((unsigned)b % (unsigned)P);
 #pragma unroll
     for (int r = 0; r < S::R; ++r) buf[swz((b - k) * S::R + k + r * P)] = t[q * S::R + r];
   }
 }
 
-template <int N, int P>
+template <int N, int P, int EL = wave_e(N)>
 __device__ __forceinline__ void pass_final(const double2 (&t)[16], double2 (&v)[16]) {
-  using S = PassShape<N, P>;
+  using S = PassShape<N, P, EL>;
 #pragma unroll
   for (int q = 0; q < S::BPL; ++q)
 #pragma unroll
@@ -557,18 +559,18 @@ __device__ __forceinline__ void pass_final(const double2 (&t)[16], double2 (&v)[
 // (this lane's vector in its wave's LDS buffer, VS entries).  Only wave-level
 // ordering is needed: no workgroup barrier.
 // ---------------------------------------------------------------------------
-template <int N, int P, int SIGN, class TW>
+template <int N, int P, int SIGN, int EL = wave_e(N), class TW>
 __device__ __forceinline__ void wave_fft_pass(double2 (&v)[16], double2* buf, int lp, const TW& tw) {
   double2 t[16];
-  pass_load<N, P>(t, buf, lp);
-  pass_compute<N, P, SIGN>(t, lp, tw);
-  if constexpr (PassShape<N, P>::LAST) {
-    pass_final<N, P>(t, v);
+  pass_load<N, P, EL>(t, buf, lp);
+  pass_compute<N, P, SIGN, EL>(t, lp, tw);
+  if constexpr (PassShape<N, P, EL>::LAST) {
+    pass_final<N, P, EL>(t, v);
   } else {
     wave_sync();
-    pass_store<N, P>(t, buf, lp);
+    pass_store<N, P, EL>(t, buf, lp);
     wave_sync();
-    wave_fft_pass<N, P * PassShape<N, P>::R, SIGN>(v, buf, lp, tw);
+    wave_fft_pass<N, P * PassShape<N, P, EL>::R, SIGN, EL>(v, buf, lp, tw);
   }
 }
 
@@ -615,9 +617,9 @@ __device__ __forceinline__ void transpose_1024_p16_p256(double2 (&t)[16]) {
   }
 }
 
-template <int N, int SIGN, class TW>
+template <int N, int SIGN, int EL = wave_e(N), class TW>
 __device__ __forceinline__ void wave_fft(double2 (&v)[16], double2* buf, int lp, const TW& tw) {
-  constexpr int E = WaveLay<N>::E;
+  constexpr int E = WaveLay<N, EL>::E;
   dft_any<E, SIGN>(v);                     // pass 1 (P = 1): butterfly lp
   if constexpr (N == 1024 && E == 16) {
     // one LDS exchange (after the register pass); the second in registers
@@ -637,7 +639,7 @@ __device__ __forceinline__ void wave_fft(double2 (&v)[16], double2* buf, int lp,
 #pragma unroll
     for (int r = 0; r < E; ++r) buf[swz(lp * E + r)] = v[r];
     wave_sync();
-    wave_fft_pass<N, E, SIGN>(v, buf, lp, tw);
+    wave_fft_pass<N, E, SIGN, EL>(v, buf, lp, tw);
     wave_sync();                           // buf may be rewritten by the caller
   }
 }
