@@ -235,13 +235,12 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
 // Column adjoint job: one field on a tile of columns (as the forward).
 // ===========================================================================
 template <int NY, int AMODE>
-__device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq,
-                                         double2* dbuf) {
+__device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0, double2* sm, const double2* twq) {
   constexpr int LPC = WaveLay<NY>::LPC, E = WaveLay<NY>::E;
   const TwN<NY> tw{twq};
   ColTile<NY> T(a, sm, i0);
   const int lp = T.lp;
-  const int wave = threadIdx.x >> 6;
+  const int wave = T.tid >> 6;
   const int i = i0 + wave * ColTile<NY>::CW + T.g;
   const bool iok = i < a.Nxh;
   const int ic = iok ? i : 0;
@@ -282,24 +281,15 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     T.store(a.lam_[FLD]);
   } else {
     // ---- cbar = (y inverse set)^T Mbar: y FFTs of the field's Mbar arrays ----
-    // Input tiles by LDS DMA (TileDMA), each issued while the previous one is
-    // transformed; the last one is the state tile of the combination below.
-    const TileDMA<NY> D{dbuf, a, i0};
-    const int cl = wave * ColTile<NY>::CW + T.g;   // this lane's tile column
+    // (tiles staged through the wave buffers; the last one loaded is the
+    // state tile of the combination below)
     const double2* state_in = AMODE == AMODE_B ? a.lam_[FLD] : a.acc_[FLD];
     if (FLD == 0) {
-      D.issue(a.mix[0]);
-      D.wait();
-      D.read(st, cl, lp);
-      __syncthreads();
-      D.issue(a.mix[1]);
+      T.stage_in(a.mix[0], st);
       wave_fft<NY, -1>(st, vb, lp, tw);
       {
         double2 v[16];
-        D.wait();
-        D.read(v, cl, lp);
-        __syncthreads();
-        D.issue(a.mix[2]);
+        T.stage_in(a.mix[1], v);
         wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) {   // psi = W / k^2
@@ -312,10 +302,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       }
       {
         double2 v[16];
-        D.wait();
-        D.read(v, cl, lp);
-        __syncthreads();
-        D.issue(state_in);
+        T.stage_in(a.mix[2], v);
         wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) {   // i ky psi -> conj: -i ky / k^2
@@ -329,17 +316,10 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       if (i0 == 0 && threadIdx.x == 0) st[0].x += w00sum;   // element (i = 0, j = 0)
     } else {
       const int m0 = FLD == 1 ? 3 : 5;
-      D.issue(a.mix[m0]);
-      D.wait();
-      D.read(st, cl, lp);
-      __syncthreads();
-      D.issue(a.mix[m0 + 1]);
+      T.stage_in(a.mix[m0], st);
       wave_fft<NY, -1>(st, vb, lp, tw);
       double2 v[16];
-      D.wait();
-      D.read(v, cl, lp);
-      __syncthreads();
-      D.issue(state_in);
+      T.stage_in(a.mix[m0 + 1], v);
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -348,15 +328,15 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       }
     }
     // ---- IFRK2 reverse combinations ----------------------------------------
-    // (the state tile, lambda or acc, is in dbuf; results go to the wave
-    // buffers for the store)
-    D.wait();
+    // (the state tile, lambda or acc, staged in the wave buffers and
+    // overwritten in place by the results for the store)
+    T.load(state_in);
     if (AMODE == AMODE_B) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
-        const double2 lamv = D.at(j, cl);
+        const double2 lamv = *sl;
         const double IFj = IFof(j);
         *sl = make_double2(IFj * (lamv.x + st[m].x), IFj * (lamv.y + st[m].y));
         st[m] = make_double2(IFj * (0.5 * dt * lamv.x + dt * st[m].x), IFj * (0.5 * dt * lamv.y + dt * st[m].y));
@@ -368,7 +348,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
         const double c = keep(j) ? 1.0 : 0.0;
-        const double2 acc = D.at(j, cl);
+        const double2 acc = *sl;
         const double2 lamv = make_double2(c * (acc.x + st[m].x), c * (acc.y + st[m].y));
         *sl = lamv;
         st[m] = make_double2(0.5 * dt * lamv.x, 0.5 * dt * lamv.y);
@@ -378,7 +358,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
-        const double2 acc = D.at(j, cl);
+        const double2 acc = vb[swz(j)];
         vb[swz(j)] = make_double2(acc.x + st[m].x, acc.y + st[m].y);
       }
       if (a.uout[FLD]) T.store(a.uout[FLD]);
@@ -456,12 +436,13 @@ __global__ __launch_bounds__(384) void k_arow(const RunDesc* __restrict__ runs, 
   }
 }
 
+// Two blocks per CU (72 KB of LDS, at most 256 registers per lane), as the
+// forward column kernels.
 template <int NY, int AMODE>
-__global__ __launch_bounds__(256) void k_acol(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
-                                              int nwork, const double2* __restrict__ tw1024) {
+__global__ __launch_bounds__(256, 2) void k_acol(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
+                                                 int nwork, const double2* __restrict__ tw1024) {
   __shared__ double2 sm[4 * (kBuf + 1)];
   __shared__ double2 twq[kTwq];
-  __shared__ double2 dbuf[4096];   // DMA tile buffer (TileDMA)
   stage_twiddles(twq, tw1024, threadIdx.x, 256);
   __syncthreads();
   const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -472,7 +453,7 @@ __global__ __launch_bounds__(256) void k_acol(const RunDesc* __restrict__ runs, 
     const int ntiles = (a.Nxh + tc - 1) / tc;
     int fld, tile;
     col_job_map(it.y, ntiles, fld, tile);   // XCD-balanced numbering, as the forward column kernels
-    acol_job<NY, AMODE>(a, fld, tile * tc, sm, twq, dbuf);
+    acol_job<NY, AMODE>(a, fld, tile * tc, sm, twq);
     __syncthreads();
   }
 }

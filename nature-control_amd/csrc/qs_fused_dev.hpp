@@ -277,6 +277,15 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
 // covers whole row segments of the tile, while the FFTs run per wavefront on
 // the lane layout "lane (lp, g) holds rows lp + LPC*m, m < E".  A tile holds
 // TC * NY = 256 * E elements: E per thread.
+// threadIdx.x as a value the compiler treats as new on every call: lane
+// offsets derived from it are recomputed per job instead of being hoisted out
+// of a persistent job loop and held (or spilled) across it.
+__device__ __forceinline__ int job_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 template <int NY, int EL = wave_e(NY)>
 struct ColTile {
   static constexpr int CW = WaveLay<NY, EL>::NV, TC = 4 * CW, LPC = WaveLay<NY, EL>::LPC, VS = WaveLay<NY, EL>::VS;
@@ -287,7 +296,7 @@ struct ColTile {
   int i0, tid, lp, g;
   double2* vb;
   __device__ __forceinline__ ColTile(const RunDesc& a_, double2* sm_, int i0_) : a(a_), sm(sm_), i0(i0_) {
-    tid = threadIdx.x;
+    tid = job_tid();
     const int wave = tid >> 6, lane = tid & 63;
     lp = lane / CW;
     g = lane % CW;
@@ -297,38 +306,43 @@ struct ColTile {
   // cooperative coalesced tile load into the wave buffers (ends with a barrier);
   // all E loads are in flight before the first LDS write (the scheduling
   // barrier keeps the compiler from pairing each load with its write)
+  // Thread t moves column c = t % TC of rows t / TC + q * (256 / TC): one
+  // base offset plus a uniform stride per instruction, recomputed per call
+  // (job_tid) so that no per-load address stays live between calls.
   __device__ __forceinline__ void load(const double2* __restrict__ src) {
+    static_assert(256 % TC == 0, "tile width divides the block");
+    __builtin_amdgcn_sched_barrier(0);   // no load hoisted above the previous phase
+    const int t = job_tid();
+    const int c = t % TC, j0 = t / TC;
+    const int ii = i0 + c;
+    const bool in = ii < a.Nxh;
+    // 32-bit element offsets (SGPR base + VGPR offset form)
+    const unsigned base = (unsigned)(j0 * a.pitch + (in ? ii : a.Nxh - 1));
+    const unsigned stride = (unsigned)((256 / TC) * a.pitch);
     double2 ld[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int e = tid + 256 * q;
-      const int c = e % TC, j = e / TC;
-      const int ii = i0 + c;
-      // 32-bit element offset: one VGPR per load (SGPR base + VGPR offset form)
-      const unsigned o = (unsigned)(j * a.pitch + (ii < a.Nxh ? ii : a.Nxh - 1));
-      ld[q] = gload(src + o);
-    }
+    for (int q = 0; q < E; ++q) ld[q] = gload(src + (base + q * stride));
     __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
+    double2* s0 = sm + (c / CW) * WS + (c % CW) * VS;
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int e = tid + 256 * q;
-      const int c = e % TC, j = e / TC;
-      sm[slot(j, c)] = i0 + c < a.Nxh ? ld[q] : make_double2(0.0, 0.0);
-    }
+    for (int q = 0; q < E; ++q) s0[swz(j0 + q * (256 / TC))] = in ? ld[q] : make_double2(0.0, 0.0);
     __syncthreads();
   }
   // cooperative coalesced tile store from the wave buffers.  Every thread reads
   // slots of every wave's buffer, so the closing barrier keeps a wave from
   // rewriting its buffer (its next FFT) while others still read it.
   __device__ __forceinline__ void store(double2* __restrict__ dst) {
+    const int t = job_tid();
+    const int c = t % TC, j0 = t / TC;
+    const int ii = i0 + c;
+    const unsigned base = (unsigned)(j0 * a.pitch + ii);
+    const unsigned stride = (unsigned)((256 / TC) * a.pitch);
+    const double2* s0 = sm + (c / CW) * WS + (c % CW) * VS;
     __syncthreads();
+    if (ii < a.Nxh) {
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int e = tid + 256 * q;
-      const int c = e % TC, j = e / TC;
-      const int ii = i0 + c;
-      if (ii < a.Nxh) gstore(dst + (unsigned)(j * a.pitch + ii), sm[slot(j, c)]);
+      for (int q = 0; q < E; ++q) gstore(dst + (base + q * stride), s0[swz(j0 + q * (256 / TC))]);
     }
     __syncthreads();
   }
@@ -367,59 +381,6 @@ __device__ __forceinline__ void col_job_map(int job, int ntiles, int& fld, int& 
     tile = full * G + r % g2;
   }
 }
-
-// Column tile of a column job in a separate LDS buffer D (row-major
-// [row][TC], TC * NY = 256 * E entries) filled by LDS DMA
-// (global_load_lds_dwordx4): the loads need no registers and complete in the
-// background, so a job issues its next input tile right after reading the
-// current one and runs the FFT meanwhile.  Instruction q of wave w writes the
-// 64 entries (w*E + q)*64 + lane: whole row segments, as ColTile::load reads
-// them.  Columns past the last one (i >= Nxh) repeat it; their results are
-// never stored.
-template <int NY, int EL = wave_e(NY)>
-struct TileDMA {
-  static constexpr int CW = WaveLay<NY, EL>::NV, TC = 4 * CW, LPC = WaveLay<NY, EL>::LPC, E = WaveLay<NY, EL>::E;
-  double2* D;
-  const RunDesc& a;
-  int i0;
-  __device__ __forceinline__ void issue(const double2* __restrict__ src) const {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if constexpr (64 % TC == 0) {
-      // a lane keeps its tile column c = lane % TC and moves 64/TC rows per
-      // instruction: one base offset plus a uniform stride
-      const int c = lane % TC;
-      const int ii = i0 + c < a.Nxh ? i0 + c : a.Nxh - 1;
-      const unsigned base = (unsigned)((wave * E * (64 / TC) + lane / TC) * a.pitch + ii);
-      const unsigned stride = (unsigned)((64 / TC) * a.pitch);
-#pragma unroll
-      for (int q = 0; q < E; ++q)
-        __builtin_amdgcn_global_load_lds((const QS_GLOBAL void*)(src + (base + q * stride)),
-                                         (__attribute__((address_space(3))) void*)(D + (wave * E + q) * 64), 16, 0,
-                                         0);
-    } else {   // tiles wider than a wave (NY <= 32): 64 consecutive columns of one row each
-#pragma unroll
-      for (int q = 0; q < E; ++q) {
-        const int e = (wave * E + q) * 64 + lane;
-        const int c = e % TC, j = e / TC;
-        const int ii = i0 + c < a.Nxh ? i0 + c : a.Nxh - 1;
-        __builtin_amdgcn_global_load_lds((const QS_GLOBAL void*)(src + (unsigned)(j * a.pitch + ii)),
-                                         (__attribute__((address_space(3))) void*)(D + (wave * E + q) * 64), 16, 0,
-                                         0);
-      }
-    }
-  }
-  // this wave's transfers done (vmcnt 0), then every wave's
-  __device__ __forceinline__ void wait() const {
-    __builtin_amdgcn_s_waitcnt(0x0f70);
-    __syncthreads();
-  }
-  // tile column c, rows lp + LPC*m, in the FFT lane layout
-  __device__ __forceinline__ void read(double2 (&v)[16], int c, int lp) const {
-#pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = D[(lp + LPC * m) * TC + c];
-  }
-  __device__ __forceinline__ double2 at(int j, int c) const { return D[j * TC + c]; }
-};
 
 }  // namespace
 }  // namespace qs

@@ -124,15 +124,17 @@ __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm
 // ===========================================================================
 // Column job: one field (FLD 0 = w, 1 = r1, 2 = r2) on a tile of 4 NV
 // columns (ColTile), one column per lane group, FFTs along y per wavefront.
+// Every input tile is staged through the wave buffers (ColTile::load) and
+// every output tile leaves through them (ColTile::store).
 template <int NY, int MODE, int EL = wave_e(NY), class TW>
 __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const TW& tw,
-                                        double2* dbuf, bool pro_arg) {
+                                        bool pro_arg) {
   // prologue: final-stage kernels only (see is_prologue)
   const bool pro = (MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && pro_arg;
   constexpr int LPC = WaveLay<NY, EL>::LPC, E = WaveLay<NY, EL>::E;
   ColTile<NY, EL> T(a, sm, i0);
   const int lp = T.lp;
-  const int i = i0 + (threadIdx.x >> 6) * ColTile<NY, EL>::CW + T.g;
+  const int i = i0 + (T.tid >> 6) * ColTile<NY, EL>::CW + T.g;
   const bool iok = i < a.Nxh;
   const int ic = iok ? i : 0;
   double2* vb = T.vb;
@@ -154,26 +156,14 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   } else {
     // ---- N_X from the forward set ------------------------------------------
     //   N_w = F0 + i ky F1 + ky^2 F2,  N_r = F_a + i ky F_b   (in st)
-    // Input tiles arrive by LDS DMA into dbuf, each issued as soon as the
-    // previous one has been read, so it loads while that one is transformed:
-    // F_a, F_b, (F2,) then the state tile of the stage update.
-    const TileDMA<NY, EL> D{dbuf, a, i0};
-    const int cl = (threadIdx.x >> 6) * ColTile<NY, EL>::CW + T.g;   // this lane's tile column
     const bool from_old = MODE == MODE_S1 || MODE == MODE_R1 || MODE == MODE_E;
     const double2* state_in = from_old ? a.old_[FLD] : a.new_[FLD];   // R2, R3: new first
     const int f0 = FLD == 0 ? 0 : (FLD == 1 ? 3 : 5);
-    D.issue(a.mix[f0]);
-    D.wait();
-    D.read(st, cl, lp);
-    __syncthreads();
-    D.issue(a.mix[f0 + 1]);
+    T.stage_in(a.mix[f0], st);
     wave_fft<NY, -1, EL>(st, vb, lp, tw);
     {
       double2 v[16];
-      D.wait();
-      D.read(v, cl, lp);
-      __syncthreads();
-      D.issue(FLD == 0 ? a.mix[2] : state_in);
+      T.stage_in(a.mix[f0 + 1], v);
       wave_fft<NY, -1, EL>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -183,10 +173,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     }
     if (FLD == 0) {
       double2 v[16];
-      D.wait();
-      D.read(v, cl, lp);
-      __syncthreads();
-      D.issue(state_in);
+      T.stage_in(a.mix[2], v);
       wave_fft<NY, -1, EL>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -205,14 +192,15 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const int j = lp + LPC * m;
       if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
     }
-    D.wait();   // the state tile is in dbuf; updated values go to the wave buffers for the store
+    // the state tile, staged in the wave buffers and updated in place
+    T.load(state_in);
     if (MODE == MODE_S1 || MODE == MODE_S2 || MODE == MODE_R1 || MODE == MODE_R4 || MODE == MODE_E) {
       // one state tile: S1 / R1 / E read old, S2 / R4 read new
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
-        const double2 in = D.at(j, cl);
+        const double2 in = *sl;
         const double2 n = st[m];
         const double2 an = make_double2(n.x * a.dt, n.y * a.dt);
         if (MODE == MODE_S1) {
@@ -258,19 +246,18 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
         double2* sl = vb + swz(j);
-        const double2 nw = D.at(j, cl);
+        const double2 nw = *sl;
         const double IFh = ifc.IFh(j);
         const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
         const double h = 1.0 / 3.0 * IFh;                                          // RK4.cuh:37-61
         *sl = make_double2(nw.x + h * bn.x, nw.y + h * bn.y);
       }
-      tile_store(a.new_[FLD]);   // its barriers: every wave has read dbuf
-      D.issue(a.old_[FLD]);
-      D.wait();
+      tile_store(a.new_[FLD]);
+      T.load(a.old_[FLD]);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int j = lp + LPC * m;
-        const double2 old = D.at(j, cl);
+        const double2 old = vb[swz(j)];
         const double IFh = ifc.IFh(j);
         const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
         if (MODE == MODE_R2) {
@@ -394,35 +381,33 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
 }
 
 template <int NY, int MODE, int EL = wave_e(NY), class TW>
-__device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const TW& tw,
-                                               double2* dbuf, bool pro) {
+__device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const TW& tw, bool pro) {
   constexpr int tc = 4 * WaveLay<NY, EL>::NV;
   const int ntiles = (a.Nxh + tc - 1) / tc;
   int fld, tile;
   col_job_map(job, ntiles, fld, tile);
-  col_job<NY, MODE, EL>(a, fld, tile * tc, sm, tw, dbuf, pro);
+  col_job<NY, MODE, EL>(a, fld, tile * tc, sm, tw, pro);
 }
 
 // Every run of a column kernel's work list has Ny = NFIX (its y FFTs; the
 // column count is a runtime value).  pro != 0: the prologue of the
 // final-stage modes (S2, R4, E).
+// Two blocks per CU, so that one block's FFTs run while the other waits on
+// its loads, stores and barriers.  Tiles are staged
+// through the wave buffers (no DMA buffer), the twiddles are computed from
+// the quarter table (no per-pass table), so a block takes 72 KB of LDS, and
+// registers are capped at 256 per lane.
 template <int NFIX, int MODE, int EL = wave_e(NFIX)>
-__global__ __launch_bounds__(256) void k_col(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
-                                             int nwork, const double2* __restrict__ tw1024, int pro) {
+__global__ __launch_bounds__(256, 2) void k_col(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
+                                                int nwork, const double2* __restrict__ tw1024, int pro) {
   __shared__ double2 sm[4 * (kBuf + 1)];
   __shared__ double2 twq[kTwq];
-  __shared__ double2 dbuf[4096];   // DMA tile buffer (TileDMA), 256 * E entries used
-  __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
   stage_twiddles(twq, tw1024, threadIdx.x, 256);
   __syncthreads();
-  if constexpr (use_twt<NFIX>()) {
-    build_twtab<NFIX>(twt, twq, threadIdx.x, 256);
-    __syncthreads();
-  }
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   for (int w = L; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
-    col_dispatch_n<NFIX, MODE, EL>(runs[it.x], it.y, sm, fused_tw<NFIX>(twq, twt), dbuf, pro != 0);
+    col_dispatch_n<NFIX, MODE, EL>(runs[it.x], it.y, sm, TwN<NFIX>{twq}, pro != 0);
     __syncthreads();
   }
 }
