@@ -55,7 +55,8 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
   constexpr int NV = L::NV, LPC = L::LPC, VS = L::VS, E = L::E, H = N / 2 + 1, RH = NV * H;
   const TwN<N> tw{twq};
   const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
+  // the wave index as a scalar: wave-uniform branches, one unrolled loop each
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;
   double2* vb = sm + wave * kBuf + g * VS;
   const double sE = a.s_ErRe;
@@ -65,41 +66,57 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
   stage_rows<N>(a.mix, a, row0, sm);
   {
     const double2* st = sm + g * H;
+    auto build = [&](auto&& pair) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int k = lp + LPC * m;
-      const int bin = k > N / 2 ? N - k : k;
-      const double kx = a.dkx * bin;
-      const double wt = (bin == 0 || 2 * bin == N) ? 1.0 : 0.5;
-      double2 X = make_double2(0.0, 0.0), Y = make_double2(0.0, 0.0);
-      if (wave == 0) {            // (Abar, Bbar) = (-i kx' F0bar, F1bar)
-        if (a.advect_w) {
+      for (int m = 0; m < E; ++m) {
+        const int k = lp + LPC * m;
+        const int bin = k > N / 2 ? N - k : k;
+        const double kx = a.dkx * bin;
+        const double wt = (bin == 0 || 2 * bin == N) ? 1.0 : 0.5;
+        double2 X = make_double2(0.0, 0.0), Y = make_double2(0.0, 0.0);
+        pair(bin, kx, X, Y);
+        v[m] = herm_pair<N>(k, make_double2(wt * X.x, wt * X.y), make_double2(wt * Y.x, wt * Y.y));
+      }
+    };
+    if (wave == 0) {            // (Abar, Bbar) = (-i kx' F0bar, F1bar)
+      const bool adv = a.advect_w;
+      build([&](int bin, double kx, double2& X, double2& Y) {
+        if (adv) {
           const double kxa = (2 * bin == N) ? 0.0 : kx;
           const double2 F0 = st[0 * RH + bin];
           X = make_double2(kxa * F0.y, -kxa * F0.x);
           Y = st[1 * RH + bin];
         }
-      } else if (wave == 1) {     // (U1bar, V1bar) = (-i kx F3bar, F4bar)
+      });
+    } else if (wave == 1) {     // (U1bar, V1bar) = (-i kx F3bar, F4bar)
+      build([&](int bin, double kx, double2& X, double2& Y) {
         const double2 F3 = st[3 * RH + bin];
         X = make_double2(kx * F3.y, -kx * F3.x);
         Y = st[4 * RH + bin];
-      } else if (wave == 2) {     // (U2bar, V2bar) = (-i kx F5bar, F6bar)
+      });
+    } else if (wave == 2) {     // (U2bar, V2bar) = (-i kx F5bar, F6bar)
+      build([&](int bin, double kx, double2& X, double2& Y) {
         const double2 F5 = st[5 * RH + bin];
         X = make_double2(kx * F5.y, -kx * F5.x);
         Y = st[6 * RH + bin];
-      } else if (wave == 3) {     // (P11bar, P12bar) = (2 i kx s F1bar, -kx^2 s F0bar)
+      });
+    } else if (wave == 3) {     // (P11bar, P12bar) = (2 i kx s F1bar, -kx^2 s F0bar)
+      build([&](int bin, double kx, double2& X, double2& Y) {
         const double2 F0 = st[0 * RH + bin], F1 = st[1 * RH + bin];
         const double t = 2.0 * kx * sE, c = -kx * kx * sE;
         X = make_double2(-t * F1.y, t * F1.x);
         Y = make_double2(c * F0.x, c * F0.y);
-      } else if (wave == 4) {     // (P21bar, -) = (s F2bar, 0)
+      });
+    } else if (wave == 4) {     // (P21bar, -) = (s F2bar, 0)
+      build([&](int bin, double, double2& X, double2&) {
         const double2 F2 = st[2 * RH + bin];
         X = make_double2(sE * F2.x, sE * F2.y);
-      } else {                    // (loc1bar, loc2bar) = (F3bar, F5bar)
+      });
+    } else {                    // (loc1bar, loc2bar) = (F3bar, F5bar)
+      build([&](int bin, double, double2& X, double2& Y) {
         X = st[3 * RH + bin];
         Y = st[5 * RH + bin];
-      }
-      v[m] = herm_pair<N>(k, make_double2(wt * X.x, wt * X.y), make_double2(wt * Y.x, wt * Y.y));
+      });
     }
   }
   __syncthreads();   // staging area free

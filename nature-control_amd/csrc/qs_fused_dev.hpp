@@ -225,41 +225,53 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
   using L = WaveLay<N, EL>;
   constexpr int NV = L::NV, LPC = L::LPC, VS = L::VS, E = L::E, H = N / 2 + 1, RH = NV * H;
   const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
+  // the wave index as a scalar: the branches below are wave-uniform, so each
+  // wave runs only its own unrolled loop, with all its LDS reads in flight
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;
   double2* vb = sm + wave * kBuf + g * VS;
   auto arr = [&](int t) -> const double2* { return sm + t * RH + g * H; };   // staged array t, vector g
-  if (wave < 5) {
+  auto build = [&](auto&& pair) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int k = lp + LPC * m;
       const int bin = k > N / 2 ? N - k : k;
-      const double kx = a.dkx * bin;
       double2 X, Y;
-      if (wave == 0) {            // (w, dy u + dx v = -W + 2 kx^2 psi + W00 [kx = 0])
-        const double2 W = arr(0)[bin], ps = arr(1)[bin];
-        const double t2 = 2.0 * kx * kx;
-        X = W;
-        Y = make_double2(-W.x + t2 * ps.x + (bin == 0 ? w00 : 0.0), -W.y + t2 * ps.y);
-      } else if (wave == 1) {     // (u = i ky psi, v = -i kx psi)
-        const double2 ps = arr(1)[bin];
-        X = arr(2)[bin];
-        Y = make_double2(kx * ps.y, -kx * ps.x);
-      } else if (wave == 2) {     // (r1, r2)
-        X = arr(3)[bin];
-        Y = arr(5)[bin];
-      } else if (wave == 3) {     // (L1, L2) = -kx^2 R + R_yy
-        const double2 R1 = arr(3)[bin], R1yy = arr(4)[bin];
-        const double2 R2 = arr(5)[bin], R2yy = arr(6)[bin];
-        X = make_double2(-kx * kx * R1.x + R1yy.x, -kx * kx * R1.y + R1yy.y);
-        Y = make_double2(-kx * kx * R2.x + R2yy.x, -kx * kx * R2.y + R2yy.y);
-      } else {                    // (dx u = i kx u, -)
-        const double2 U = arr(2)[bin];
-        X = make_double2(-kx * U.y, kx * U.x);
-        Y = make_double2(0.0, 0.0);
-      }
+      pair(bin, a.dkx * bin, X, Y);
       v[m] = herm_pair<N>(k, X, Y);
     }
+  };
+  if (wave == 0) {          // (w, dy u + dx v = -W + 2 kx^2 psi + W00 [kx = 0])
+    build([&](int bin, double kx, double2& X, double2& Y) {
+      const double2 W = arr(0)[bin], ps = arr(1)[bin];
+      const double t2 = 2.0 * kx * kx;
+      X = W;
+      Y = make_double2(-W.x + t2 * ps.x + (bin == 0 ? w00 : 0.0), -W.y + t2 * ps.y);
+    });
+  } else if (wave == 1) {   // (u = i ky psi, v = -i kx psi)
+    build([&](int bin, double kx, double2& X, double2& Y) {
+      const double2 ps = arr(1)[bin];
+      X = arr(2)[bin];
+      Y = make_double2(kx * ps.y, -kx * ps.x);
+    });
+  } else if (wave == 2) {   // (r1, r2)
+    build([&](int bin, double, double2& X, double2& Y) {
+      X = arr(3)[bin];
+      Y = arr(5)[bin];
+    });
+  } else if (wave == 3) {   // (L1, L2) = -kx^2 R + R_yy
+    build([&](int bin, double kx, double2& X, double2& Y) {
+      const double2 R1 = arr(3)[bin], R1yy = arr(4)[bin];
+      const double2 R2 = arr(5)[bin], R2yy = arr(6)[bin];
+      X = make_double2(-kx * kx * R1.x + R1yy.x, -kx * kx * R1.y + R1yy.y);
+      Y = make_double2(-kx * kx * R2.x + R2yy.x, -kx * kx * R2.y + R2yy.y);
+    });
+  } else if (wave == 4) {   // (dx u = i kx u, -)
+    build([&](int bin, double kx, double2& X, double2& Y) {
+      const double2 U = arr(2)[bin];
+      X = make_double2(-kx * U.y, kx * U.x);
+      Y = make_double2(0.0, 0.0);
+    });
   }
   __syncthreads();   // staging area free
   if (wave < 5) {

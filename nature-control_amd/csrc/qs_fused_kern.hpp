@@ -349,7 +349,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
   }
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   constexpr int NV = WaveLay<NFIX, EL>::NV;
-  if (threadIdx.x >= 384) {
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) >= 6) {   // wave-uniform
     // Loader waves.  Per job they pass exactly the compute waves' block
     // barriers: staging ready, the kRowBodyBarriers of row_body, job done.
     const int lt = threadIdx.x - 384;
