@@ -360,12 +360,12 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
     }
     for (int w = L; w < nwork; w += gridDim.x) {
       rl.commit(sm, rsc, lt);            // this job's rows (waits for the loads)
-      const int wn = w + gridDim.x;
-      if (wn < nwork) {
-        const int2 it = work[wn];
-        rl.issue(runs[it.x], it.y * NV, lt);   // the next job's, in flight during this one
-      }
       __syncthreads();   // staging ready
+      const int wn = w + gridDim.x;      // the next job's rows, in flight during this one
+      if (wn < nwork) {                  // (issued after the barrier: the compute waves start at once)
+        const int2 it = work[wn];
+        rl.issue(runs[it.x], it.y * NV, lt);
+      }
 #pragma unroll 1
       for (int b = 0; b < kRowBodyBarriers; ++b) __syncthreads();
       __syncthreads();   // job done
