@@ -266,7 +266,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
   const double dt = a.dt;
   auto kyj = [&](int j) { return a.dky * (j < NY / 2 + 1 ? j : j - NY); };
   auto keep = [&](int j) { return (i < ((a.Nxh - 1) * 2 / 3 + 1)) && ((j < NY / 3 + 1) || ((NY - j) < NY / 3)); };
-  const IFac ifc(a, FLD, kx, NY);   // the forward's integrating factor, exp(L dt)
+  const IFac ifc(a, FLD, ic, NY);   // the forward's integrating factor, exp(L dt)
   auto IFof = [&](int j) { return ifc.IF(j); };
 
   // W(0,0) enters every row's dy u + dx v at bin 0: its adjoint is the sum of

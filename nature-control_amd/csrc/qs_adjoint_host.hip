@@ -204,10 +204,11 @@ int copy_pitched(const qs_solver* S, double2* dst, const double2* src, hipStream
   return QS_OK;
 }
 
-int launch_row(AdjGroup* G, const RunDesc* d, hipStream_t st) {   // persistent: one block per CU
+// fin != 0: stage 2 follows (see launch_fused_row)
+int launch_row(AdjGroup* G, const RunDesc* d, hipStream_t st, int fin) {   // persistent: one block per CU
   const Mesh* m = mesh_of(G);
   const int grid = m->priv->ncu > 0 ? std::min(G->nrow, m->priv->ncu) : G->nrow;
-  return launch_fused_row(m->Nx, d, G->d_row, G->nrow, grid, m->priv->tw1024, st);
+  return launch_fused_row(m->Nx, d, G->d_row, G->nrow, grid, m->priv->tw1024, st, fin);
 }
 int launch_col(AdjGroup* G, const RunDesc* d, int mode, hipStream_t st) {
   const Mesh* m = mesh_of(G);
@@ -307,9 +308,9 @@ int record_segment(AdjGroup* G, int seg, hipStream_t st) {
   auto at = [&](int s) { return D + (size_t)s * G->R; };
   QS_TRY(launch_col(G, at(0), MODE_PRO, st));
   for (int n = 0; n < seg_len(G, seg); ++n) {
-    QS_TRY(launch_row(G, at(1 + 2 * n), st));
+    QS_TRY(launch_row(G, at(1 + 2 * n), st, 0));
     QS_TRY(launch_col(G, at(1 + 2 * n), MODE_S1, st));
-    QS_TRY(launch_row(G, at(2 + 2 * n), st));
+    QS_TRY(launch_row(G, at(2 + 2 * n), st, 1));
     QS_TRY(launch_col(G, at(2 + 2 * n), MODE_S2, st));
   }
   return QS_OK;
@@ -357,9 +358,9 @@ int group_forward(AdjGroup* G, int nsteps, int k, const double* alpha_t) {
     if (n % k == 0) QS_TRY(checkpoint(n / k));
     const RunDesc* d = G->d_desc + (size_t)n * R;
     if (n == 0) QS_TRY(launch_col(G, d, MODE_PRO, st));
-    QS_TRY(launch_row(G, d, st));
+    QS_TRY(launch_row(G, d, st, 0));
     QS_TRY(launch_col(G, d, MODE_S1, st));
-    QS_TRY(launch_row(G, d, st));
+    QS_TRY(launch_row(G, d, st, 1));
     QS_TRY(launch_col(G, d, MODE_S2, st));
   }
   QS_TRY(checkpoint(last));

@@ -120,7 +120,7 @@ int class_step(qs_ensemble* E, int ci, hipStream_t st) {
   const int el = E->cfg.lanes[ci];
   QS_TRY(launch_fused_row_l(c.N, el, E->d_desc, c.d_row, c.nrow, rb, tw, st));
   QS_TRY(launch_fused_col_l(c.N, el, MODE_S1, E->d_desc, c.d_col, c.ncol, cb, tw, st));
-  QS_TRY(launch_fused_row_l(c.N, el, E->d_desc, c.d_row, c.nrow, rb, tw, st));
+  QS_TRY(launch_fused_row_l(c.N, el, E->d_desc, c.d_row, c.nrow, rb, tw, st, 1));
   QS_TRY(launch_fused_col_l(c.N, el, MODE_S2, E->d_desc, c.d_col, c.ncol, cb, tw, st));
   return QS_OK;
 }

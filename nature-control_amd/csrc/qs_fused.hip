@@ -37,14 +37,18 @@
 namespace qs {
 
 namespace {
-__global__ __launch_bounds__(256) void k_ifac_rows(double* ify, int Ny, double dky, double Re, double DQ, double dt) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < Ny) ifac_row(j, Ny, dky, Re, DQ, dt, ify);
+__global__ __launch_bounds__(256) void k_ifac(double* ify, int Nxh, int Ny, double dkx, double dky, double Re,
+                                              double Rf, double DQ, double aQ, double dt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < Ny) ifac_row(t, Ny, dky, Re, DQ, dt, ify);
+  if (t < Nxh) ifac_col(t, Nxh, Ny, dkx, Re, Rf, DQ, aQ, dt, ify);
 }
 }  // namespace
 
-int ifac_rows_launch(double* ify, int Ny, double dky, double Re, double DQ, double dt, hipStream_t st) {
-  hipLaunchKernelGGL(k_ifac_rows, dim3((Ny + 255) / 256), dim3(256), 0, st, ify, Ny, dky, Re, DQ, dt);
+int ifac_launch(double* ify, int Nxh, int Ny, double dkx, double dky, double Re, double Rf, double DQ, double aQ,
+                double dt, hipStream_t st) {
+  const int n = Ny > Nxh ? Ny : Nxh;
+  hipLaunchKernelGGL(k_ifac, dim3((n + 255) / 256), dim3(256), 0, st, ify, Nxh, Ny, dkx, dky, Re, Rf, DQ, aQ, dt);
   QS_CHECK_LAUNCH();
   return QS_OK;
 }
@@ -74,8 +78,8 @@ int col_jobs(const RunDesc& a, int el) {
   }
 
 int launch_fused_row(int nfix, const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw1024,
-                     hipStream_t st) {
-  QS_NFIX_SWITCH(nfix, launch_row_n, runs, work, nwork, grid, tw1024, st);
+                     hipStream_t st, int fin) {
+  QS_NFIX_SWITCH(nfix, launch_row_n, runs, work, nwork, grid, tw1024, st, fin);
 }
 
 int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, int nwork, int grid,

@@ -35,7 +35,7 @@ struct RunDesc {
   double* w00;                    // W(0,0) of the stage the row pass reads
   double* w00o;                   // W(0,0) of the stage the column kernel writes
   int* flag;                      // non-finite flag
-  const double* ify;              // [4][Ny] row factors of the integrating factors (IFac)
+  const double* ify;              // integrating-factor tables (IFac): [4][Ny] rows, then [4][Nxh] columns
   double dkx, dky;                // 2 pi / Lx, 2 pi / Ly: kx(i) = dkx i, ky(j) = dky (j or j - Ny)
   double alpha, lambda, DQ, aQ, cQ, s_ErRe, scale, dt, Re, Rf;
   int advect_w;
@@ -58,8 +58,10 @@ int col_jobs(const RunDesc& a, int el = 0);
 // nfix: the size common to all listed runs in the kernel's FFT direction (Nx
 // for the row pass, Ny for the column stages): a power of two 16..1024 or
 // 3 * 2^a 48..768; any other value is QS_ERR_UNSUPPORTED_SIZE.
+// fin != 0: the launch is followed by a final column stage (IFRK2 stage 2, RK4
+// stage 4, IF-Euler), so only the forward-set bins that stage reads are formed.
 int launch_fused_row(int nfix, const RunDesc* runs, const int2* work, int nwork, int grid,
-                     const double2* tw1024, hipStream_t st);
+                     const double2* tw1024, hipStream_t st, int fin = 0);
 int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, int nwork, int grid,
                      const double2* tw1024, hipStream_t st);
 
@@ -68,7 +70,7 @@ int launch_fused_col(int nfix, int mode, const RunDesc* runs, const int2* work, 
 // sides 64, 128, 256 and 512 (lane_layouts) and the modes of the IFRK2 step
 // (MODE_PRO, MODE_S1, MODE_S2); anything else is QS_ERR_UNSUPPORTED_SIZE.
 int launch_fused_row_l(int nfix, int el, const RunDesc* runs, const int2* work, int nwork, int grid,
-                       const double2* tw1024, hipStream_t st);
+                       const double2* tw1024, hipStream_t st, int fin = 0);
 int launch_fused_col_l(int nfix, int el, int mode, const RunDesc* runs, const int2* work, int nwork, int grid,
                        const double2* tw1024, hipStream_t st);
 // Layouts (elements per lane) available for side n, the default first; returns
@@ -92,10 +94,13 @@ int wave_fft_test(int n, int sign, const double2* in, double2* out, const double
 
 inline int pitch_of(int Nxh) { return (Nxh + 7) & ~7; }
 
-// The integrating-factor row factors of IFac (row q of [4][Ny]: 0 w with
-// h = dt, 1 Q with dt, 2 w with dt/2, 3 Q with dt/2): exp(c2 ky(j)^2 h), c2 =
-// -1/Re for w, -D_Q for Q.  One device routine for every table, so that runs
-// built by different paths compute identical factors.
+// The integrating factors exp(L h) of IFac as separable tables, L = c2 (kx^2 +
+// ky^2) + c0 (c2 = -1/Re, c0 = -Rf/Re for w; c2 = -D_Q, c0 = a_Q for Q):
+//   rows    ify[q Ny + j]           = exp(c2 ky(j)^2 h),
+//   columns ify[4 Ny + q Nxh + i]   = exp((c2 kx(i)^2 + c0) h),
+// q = 0: w with h = dt, 1: Q with dt, 2: w with dt/2, 3: Q with dt/2.  One
+// device routine per table, so that runs built by different paths compute
+// identical factors.
 __device__ __forceinline__ void ifac_row(int j, int Ny, double dky, double Re, double DQ, double dt, double* ify) {
   const double ky = dky * (j < Ny / 2 + 1 ? j : j - Ny);
   const double cw = -1.0 / Re * ky * ky, cq = -DQ * ky * ky;
@@ -104,8 +109,22 @@ __device__ __forceinline__ void ifac_row(int j, int Ny, double dky, double Re, d
   ify[2 * Ny + j] = exp(cw * (dt * 0.5));
   ify[3 * Ny + j] = exp(cq * (dt * 0.5));
 }
-// Fills a solver's table on stream st.
-int ifac_rows_launch(double* ify, int Ny, double dky, double Re, double DQ, double dt, hipStream_t st);
+__device__ __forceinline__ void ifac_col(int i, int Nxh, int Ny, double dkx, double Re, double Rf, double DQ,
+                                         double aQ, double dt, double* ify) {
+  const double kx = dkx * i;
+  const double lw = -1.0 / Re * kx * kx + -Rf / Re;
+  const double lq = -DQ * kx * kx + aQ;
+  double* t = ify + 4 * Ny;
+  t[i] = exp(lw * dt);
+  t[Nxh + i] = exp(lq * dt);
+  t[2 * Nxh + i] = exp(lw * (dt * 0.5));
+  t[3 * Nxh + i] = exp(lq * (dt * 0.5));
+}
+// Length of the tables of an Nxh x Ny half spectrum (doubles).
+inline size_t ifac_len(int Nxh, int Ny) { return 4 * ((size_t)Ny + (size_t)Nxh); }
+// Fills a solver's tables on stream st.
+int ifac_launch(double* ify, int Nxh, int Ny, double dkx, double dky, double Re, double Rf, double DQ, double aQ,
+                double dt, hipStream_t st);
 
 
 }  // namespace qs

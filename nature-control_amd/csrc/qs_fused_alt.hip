@@ -47,10 +47,10 @@ int lane_layouts(int n, int* el) {
 }
 
 int launch_fused_row_l(int nfix, int el, const RunDesc* runs, const int2* work, int nwork, int grid,
-                       const double2* tw1024, hipStream_t st) {
-  if (el == 0 || el == wave_e(nfix)) return launch_fused_row(nfix, runs, work, nwork, grid, tw1024, st);
+                       const double2* tw1024, hipStream_t st, int fin) {
+  if (el == 0 || el == wave_e(nfix)) return launch_fused_row(nfix, runs, work, nwork, grid, tw1024, st, fin);
 #define QS_ALT_ROW(NN, EE) \
-  if (nfix == NN && el == EE) return launch_row_n<NN, EE>(runs, work, nwork, grid, tw1024, st);
+  if (nfix == NN && el == EE) return launch_row_n<NN, EE>(runs, work, nwork, grid, tw1024, st, fin);
   QS_ALT_LAYOUTS(QS_ALT_ROW)
 #undef QS_ALT_ROW
   return fail(QS_ERR_UNSUPPORTED_SIZE, "row kernel: lane layout %d at side %d", el, nfix);

@@ -113,20 +113,19 @@ __device__ __forceinline__ double recip(double x) {
 // Integrating factors exp(L dt) and exp(L dt / 2) of the column kernels
 // (friction/Qsolver.cuh:20-54 with the D_Q, a_Q coefficients), separably:
 // L = c2 (kx^2 + ky^2) + c0 with c2 = -1/Re, c0 = -Rf/Re for w and c2 = -D_Q,
-// c0 = a_Q for r1, r2, so exp(L h) = exp((c2 kx^2 + c0) h) exp(c2 ky^2 h): one
-// exp per lane (its column) per factor, and the row factor from the run's
-// table a.ify (row q of [4][Ny]: 0 w with h = dt, 1 Q dt, 2 w dt/2, 3 Q dt/2,
-// built by the solver).  Within a few ulp of exp(L h).
+// c0 = a_Q for r1, r2, so exp(L h) = exp((c2 kx^2 + c0) h) exp(c2 ky^2 h), the
+// product of a column factor and a row factor from the run's tables a.ify
+// (ifac_row, ifac_col; built once per solver).  Within a few ulp of exp(L h).
+// No exp is evaluated in the kernels: its constants, hoisted out of the job
+// loop, had been spilled.
 struct IFac {
   double fx, fxh;
   const double* fy;
   const double* fyh;
-  __device__ __forceinline__ IFac(const RunDesc& a, int FLD, double kx, int NY) {
-    const double c2 = FLD == 0 ? -1.0 / a.Re : -a.DQ;
-    const double c0 = FLD == 0 ? -a.Rf / a.Re : a.aQ;
-    const double l = c2 * kx * kx + c0;
-    fx = exp(l * a.dt);
-    fxh = exp(l * (a.dt * 0.5));
+  __device__ __forceinline__ IFac(const RunDesc& a, int FLD, int i, int NY) {
+    const double* cx = a.ify + 4 * NY;
+    fx = gload(cx + (FLD == 0 ? 0 : a.Nxh) + i);
+    fxh = gload(cx + (FLD == 0 ? 2 : 3) * a.Nxh + i);
     fy = a.ify + (FLD == 0 ? 0 : NY);
     fyh = a.ify + (FLD == 0 ? 2 * NY : 3 * NY);
   }
@@ -297,6 +296,13 @@ __device__ __forceinline__ int job_tid() {
   asm volatile("" : "+v"(t));
   return t;
 }
+// x as a value the compiler treats as new at every call: what is derived from
+// it (wavenumbers, integrating factors) is recomputed where it is used instead
+// of being kept live, or spilled, across the transforms of a column job.
+__device__ __forceinline__ int opq(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
 
 template <int NY, int EL = wave_e(NY)>
 struct ColTile {
@@ -358,15 +364,19 @@ struct ColTile {
     }
     __syncthreads();
   }
+  // (the lane's LDS slots are recomputed per call from an opaque lane index, not
+  // kept live across the job's transforms)
   __device__ __forceinline__ void stage_in(const double2* __restrict__ src, double2 (&v)[16]) {
     load(src);
+    const int l = opq(lp);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = vb[swz(lp + LPC * m)];
+    for (int m = 0; m < E; ++m) v[m] = vb[swz(l + LPC * m)];
     wave_sync();
   }
   __device__ __forceinline__ void stage_out(double2* __restrict__ dst, const double2 (&v)[16]) {
+    const int l = opq(lp);
 #pragma unroll
-    for (int m = 0; m < E; ++m) vb[swz(lp + LPC * m)] = v[m];
+    for (int m = 0; m < E; ++m) vb[swz(l + LPC * m)] = v[m];
     store(dst);
   }
 };
@@ -391,6 +401,46 @@ __device__ __forceinline__ void col_job_map(int job, int ntiles, int& fld, int& 
     const int r = job - full * 3 * G;
     fld = r / g2;
     tile = full * G + r % g2;
+  }
+}
+
+// The final column stages (IFRK2 stage 2, RK4 stage 4, IF-Euler) end with the
+// dealias mask (Qsolver.cuh:314-322), which zeroes every column i >= kcut =
+// (Nxh - 1) * 2 / 3 + 1.  A tile whose first column is >= kcut therefore holds
+// only zeros after the stage, state and next inverse set alike: its job
+// stores zeros and runs no transform (col_zero_job).
+__device__ __forceinline__ int col_kcut(const RunDesc& a) { return (a.Nxh - 1) * 2 / 3 + 1; }
+// x-bins of the forward set that a final column stage reads: the columns of
+// its tiles that start below kcut.  The tile width of every column layout of
+// side Ny divides 4096 / Ny (powers of two) or 3072 / Ny (3 * 2^a), so kcut
+// rounded up to that multiple covers them all.
+__device__ __forceinline__ int col_kept_cols(const RunDesc& a) {
+  const int T = (a.Ny % 3 == 0 ? 3072 : 4096) / a.Ny;
+  const int kc = T > 0 ? (col_kcut(a) + T - 1) / T * T : a.Nxh;
+  return kc < a.Nxh ? kc : a.Nxh;
+}
+
+// Column job index -> (field, tile) of a final stage, nkeep = tiles that start
+// below kcut.  The eight XCD ranges of the numbering (xcd_remap) each get an
+// even share of the transform-carrying tiles and of the zero tiles, each share
+// contiguous in tile index: group x lists the vorticity, r1 and r2 jobs of its
+// kept tiles, then those of its zero tiles.  (The stage-1 numbering would
+// leave the last XCDs only zero tiles.)
+__device__ __forceinline__ void col_job_map_final(int job, int ntiles, int nkeep, int& fld, int& tile) {
+  const int nz = ntiles - nkeep;
+  int x = 0;   // the last group starting at or before job (groups may be empty)
+  for (int g = 1; g < 8; ++g)
+    if (job >= 3 * ((g * nkeep) / 8 + (g * nz) / 8)) x = g;
+  const int h0 = (x * nkeep) / 8, hc = ((x + 1) * nkeep) / 8 - h0;
+  const int z0 = (x * nz) / 8, zc = ((x + 1) * nz) / 8 - z0;
+  int r = job - 3 * (h0 + z0);
+  if (r < 3 * hc) {
+    fld = r / hc;
+    tile = h0 + r % hc;
+  } else {
+    r -= 3 * hc;
+    fld = r / zc;
+    tile = nkeep + z0 + r % zc;
   }
 }
 

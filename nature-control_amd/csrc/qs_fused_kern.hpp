@@ -29,13 +29,11 @@ namespace {
 constexpr int kRowBodyBarriers = 4;
 template <int N, int EL = wave_e(N), class TW>
 __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm, const TW& tw, double w00,
-                                         double alu) {
+                                         double alu, int kout) {
   using L = WaveLay<N, EL>;
   constexpr int NV = L::NV;             // rows of this job (= vectors per wave FFT)
   constexpr int LPC = L::LPC, E = L::E;
   constexpr int VS = L::VS;             // padded vector stride inside a wave buffer
-  constexpr int H = N / 2 + 1;
-  constexpr int RH = NV * H;            // staged bins per array
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;     // lane = lp * NV + g
@@ -87,9 +85,10 @@ __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm
   __syncthreads();
 
   // ---- 6. unpack the two-for-one spectra and form F0..F6 ---------------------
+  // (bins k < kout: all H, or those a final column stage reads)
   const double sE = a.s_ErRe;
-  for (int e = tid; e < RH; e += 384) {
-    const int vec = e / H, k = e % H;
+  for (int e = tid; e < NV * kout; e += 384) {
+    const int vec = e / kout, k = e % kout;
     const int y = row0 + vec;
     if (y >= a.Ny) continue;
     constexpr bool POW2 = (N & (N - 1)) == 0;
@@ -167,7 +166,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       wave_fft<NY, -1, EL>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const double ky = kyj(lp + LPC * m);
+        const double ky = kyj(opq(lp) + LPC * m);
         st[m] = make_double2(st[m].x - ky * v[m].y, st[m].y + ky * v[m].x);
       }
     }
@@ -177,19 +176,19 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       wave_fft<NY, -1, EL>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const double ky = kyj(lp + LPC * m);
+        const double ky = kyj(opq(lp) + LPC * m);
         st[m] = make_double2(st[m].x + ky * ky * v[m].x, st[m].y + ky * ky * v[m].y);
       }
     }
     // ---- stage update (state tiles read and rewritten in LDS) ---------------
     // N_X is in st.  Per element: IF = exp(L dt), IFh = exp(L dt / 2) (IFac).
-    const IFac ifc(a, FLD, kx, NY);
+    const IFac ifc(a, FLD, ic, NY);
     auto keep = [&](int j) {   // dealias mask, Qsolver.cuh:314-322
       return (i < ((a.Nxh - 1) * 2 / 3 + 1)) && ((j < NY / 3 + 1) || ((NY - j) < NY / 3));
     };
 #pragma unroll
     for (int m = 0; m < E; ++m) {   // real part of N_w at the self-conjugate bins
-      const int j = lp + LPC * m;
+      const int j = opq(lp) + LPC * m;
       if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
     }
     // the state tile, staged in the wave buffers and updated in place
@@ -198,7 +197,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       // one state tile: S1 / R1 / E read old, S2 / R4 read new
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const int j = lp + LPC * m;
+        const int j = opq(lp) + LPC * m;
         double2* sl = vb + swz(j);
         const double2 in = *sl;
         const double2 n = st[m];
@@ -244,7 +243,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       // R2, R3: new += 1/3 IFh dt N (new tile), then curr from old (old tile)
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const int j = lp + LPC * m;
+        const int j = opq(lp) + LPC * m;
         double2* sl = vb + swz(j);
         const double2 nw = *sl;
         const double IFh = ifc.IFh(j);
@@ -256,7 +255,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       T.load(a.old_[FLD]);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const int j = lp + LPC * m;
+        const int j = opq(lp) + LPC * m;
         const double2 old = vb[swz(j)];
         const double IFh = ifc.IFh(j);
         const double2 bn = make_double2(st[m].x * a.dt, st[m].y * a.dt);
@@ -283,7 +282,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {   // st <- psi
-      const int j = lp + LPC * m;
+      const int j = opq(lp) + LPC * m;
       const double ky = kyj(j);
       const double k2 = kx * kx + ky * ky;
       st[m] = (i == 0 && j == 0) || !iok ? make_double2(0.0, 0.0) : make_double2(st[m].x / k2, st[m].y / k2);
@@ -297,7 +296,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {   // st <- i ky psi
-      const double ky = kyj(lp + LPC * m);
+      const double ky = kyj(opq(lp) + LPC * m);
       st[m] = make_double2(-ky * st[m].y, ky * st[m].x);
     }
     wave_fft<NY, +1, EL>(st, vb, lp, tw);
@@ -313,7 +312,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {   // st <- -ky^2 R
-      const double ky = kyj(lp + LPC * m);
+      const double ky = kyj(opq(lp) + LPC * m);
       st[m] = make_double2(-ky * ky * st[m].x, -ky * ky * st[m].y);
     }
     wave_fft<NY, +1, EL>(st, vb, lp, tw);
@@ -332,9 +331,11 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
 // (RowLoader) per block.
 constexpr int kRowThreads = 512;
 
+// fin != 0: a final column stage follows, which reads only the x-bins below
+// col_kept_cols of the forward set; the others are not formed.
 template <int NFIX, int EL = wave_e(NFIX)>
 __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__ runs, const int2* __restrict__ work,
-                                                    int nwork, const double2* __restrict__ tw1024) {
+                                                    int nwork, const double2* __restrict__ tw1024, int fin) {
   static_assert(NFIX > 0, "one instantiation per x size");
   constexpr int NT = kRowThreads;
   __shared__ double2 sm[6 * kBuf];
@@ -374,9 +375,35 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
     for (int w = L; w < nwork; w += gridDim.x) {
       const int2 it = work[w];          // (run, job)
       __syncthreads();   // staging ready
-      row_body<NFIX, EL>(runs[it.x], it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1]);
+      const RunDesc& a = runs[it.x];
+      row_body<NFIX, EL>(a, it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1],
+                         fin ? col_kept_cols(a) : NFIX / 2 + 1);
       __syncthreads();   // job done: wave buffers free for the next staging
     }
+  }
+}
+
+// Job of a final stage on a tile that starts at or beyond kcut (every column
+// masked): the state and the field's inverse-set arrays get zeros, which is
+// what the stage computes there from finite values (and all that later
+// kernels read).  Coalesced as ColTile::store: thread t writes column
+// i0 + t % TC of rows t / TC + q (256 / TC).
+template <int NY, int EL = wave_e(NY)>
+__device__ __forceinline__ void col_zero_job(const RunDesc& a, int fld, int i0) {
+  using T = ColTile<NY, EL>;
+  const int t = job_tid();
+  const int c = t % T::TC, j0 = t / T::TC, ii = i0 + c;
+  if (ii >= a.Nxh) return;
+  const unsigned base = (unsigned)(j0 * a.pitch + ii);
+  const unsigned stride = (unsigned)((256 / T::TC) * a.pitch);
+  const int m0 = fld == 0 ? 0 : (fld == 1 ? 3 : 5), nm = fld == 0 ? 3 : 2;
+  const double2 z = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int q = 0; q < T::E; ++q) gstore(a.old_[fld] + (base + q * stride), z);
+  for (int k = 0; k < nm; ++k) {
+    double2* dst = a.mout[m0 + k];
+#pragma unroll
+    for (int q = 0; q < T::E; ++q) gstore(dst + (base + q * stride), z);
   }
 }
 
@@ -385,7 +412,17 @@ __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double
   constexpr int tc = 4 * WaveLay<NY, EL>::NV;
   const int ntiles = (a.Nxh + tc - 1) / tc;
   int fld, tile;
-  col_job_map(job, ntiles, fld, tile);
+  if ((MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && !pro) {
+    const int nk = (col_kcut(a) + tc - 1) / tc;
+    const int nkeep = nk < ntiles ? nk : ntiles;
+    col_job_map_final(job, ntiles, nkeep, fld, tile);
+    if (tile >= nkeep) {
+      col_zero_job<NY, EL>(a, fld, tile * tc);
+      return;
+    }
+  } else {
+    col_job_map(job, ntiles, fld, tile);
+  }
   col_job<NY, MODE, EL>(a, fld, tile * tc, sm, tw, pro);
 }
 
@@ -413,8 +450,9 @@ __global__ __launch_bounds__(256, 2) void k_col(const RunDesc* __restrict__ runs
 }
 
 template <int NFIX, int EL = wave_e(NFIX)>
-int launch_row_n(const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw, hipStream_t st) {
-  hipLaunchKernelGGL((k_row<NFIX, EL>), dim3(grid), dim3(kRowThreads), 0, st, runs, work, nwork, tw);
+int launch_row_n(const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw, hipStream_t st,
+                 int fin) {
+  hipLaunchKernelGGL((k_row<NFIX, EL>), dim3(grid), dim3(kRowThreads), 0, st, runs, work, nwork, tw, fin);
   QS_CHECK_LAUNCH();
   return QS_OK;
 }

@@ -29,7 +29,8 @@ constexpr int kArrays = 10;                  // new_[3] + mix[7]
 constexpr long kArraySlots = kArrays * kMaxSpec;
 constexpr int kMaxWork = 1024;               // work-list entries (row and column jobs)
 constexpr int kMaxSide = 1024;               // longest Ny (integrating-factor rows)
-constexpr long kTailBytes = 16384 + 4 * kMaxSide * 8;   // descriptor, work list, W(0,0), flag; IF rows
+// descriptor, work list, W(0,0), flag; integrating-factor tables (rows and columns)
+constexpr long kTailBytes = 16384 + 4 * (kMaxSide + kMaxSide / 2 + 1) * 8;
 static_assert(((sizeof(RunDesc) + 255) & ~(size_t)255) + sizeof(int2) * kMaxWork + 16 + 4 * kMaxSide * 8 <=
                   (size_t)kTailBytes,
               "scratch tail");
@@ -61,16 +62,18 @@ int scratch_ptr(double2** out) {
 }
 
 // Descriptor into the scratch tail, the identity work list (run 0, job k),
-// the integrating-factor rows (those of ifac_rows), and the non-finite flag
-// cleared.
+// the integrating-factor tables (those of ifac_launch), and the non-finite
+// flag cleared.
 __global__ __launch_bounds__(256) void k_noalloc_setup(RunDesc d, RunDesc* dd, int2* work, int nwork, int* flag,
-                                                       double* ify, int Ny) {
+                                                       double* ify) {
   if (threadIdx.x == 0) {
     *dd = d;
     *flag = 0;
   }
   for (int k = threadIdx.x; k < nwork; k += blockDim.x) work[k] = make_int2(0, k);
-  for (int j = threadIdx.x; j < Ny; j += blockDim.x) ifac_row(j, Ny, d.dky, d.Re, d.DQ, d.dt, ify);
+  for (int j = threadIdx.x; j < d.Ny; j += blockDim.x) ifac_row(j, d.Ny, d.dky, d.Re, d.DQ, d.dt, ify);
+  for (int i = threadIdx.x; i < d.Nxh; i += blockDim.x)
+    ifac_col(i, d.Nxh, d.Ny, d.dkx, d.Re, d.Rf, d.DQ, d.aQ, d.dt, ify);
 }
 
 __global__ void k_noalloc_flag(const int* flag, int* out) { *out = *flag; }
@@ -115,7 +118,7 @@ extern "C" int qs_step_ifrk2_noalloc(const Mesh* mesh, const qs_params* p, const
   int2* work = (int2*)(tail + ((sizeof(RunDesc) + 255) & ~(size_t)255));
   double* w00 = (double*)(work + kMaxWork);
   int* flag = (int*)(w00 + 1);
-  double* ify = w00 + 2;   // [4][Ny]
+  double* ify = w00 + 2;   // ifac_len(Nxh, Ny)
   d.Ra = Ra;
   d.alpha_p = nullptr;
   d.w00 = d.w00o = w00;
@@ -138,7 +141,7 @@ extern "C" int qs_step_ifrk2_noalloc(const Mesh* mesh, const qs_params* p, const
   const double2* tw = Nx % 3 == 0 ? mesh->priv->tw768 : mesh->priv->tw1024;
   const hipStream_t s = stream_of(mesh);
 
-  hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(256), 0, s, d, dd, work, nwork, flag, ify, Ny);
+  hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(256), 0, s, d, dd, work, nwork, flag, ify);
   QS_CHECK_LAUNCH();
   // The inverse set of the caller's state (the scratch keeps nothing from an
   // earlier call), then the steps: the same kernels and arithmetic as
@@ -147,7 +150,7 @@ extern "C" int qs_step_ifrk2_noalloc(const Mesh* mesh, const qs_params* p, const
   for (int n = 0; n < nsteps; ++n) {
     QS_TRY(launch_fused_row(Nx, dd, work, nrow, rgrid, tw, s));
     QS_TRY(launch_fused_col(Ny, MODE_S1, dd, work, ncol, ncol, tw, s));
-    QS_TRY(launch_fused_row(Nx, dd, work, nrow, rgrid, tw, s));
+    QS_TRY(launch_fused_row(Nx, dd, work, nrow, rgrid, tw, s, 1));
     QS_TRY(launch_fused_col(Ny, MODE_S2, dd, work, ncol, ncol, tw, s));
   }
   if (nonfinite) {

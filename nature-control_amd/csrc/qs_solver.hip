@@ -77,10 +77,11 @@ int fused_check(const qs_solver* S) {
 }
 // The staged twiddle table of the fused kernels: W_768 for 3 * 2^a sides.
 const double2* fused_tw(const Mesh* m) { return m->Nx % 3 == 0 ? m->priv->tw768 : m->priv->tw1024; }
-int row_rhs_raw(qs_solver* S) {
+// fin != 0: a final column stage follows (see launch_fused_row).
+int row_rhs_raw(qs_solver* S, int fin) {
   QS_TRY(fused_check(S));
   return launch_fused_row(S->mesh->Nx, S->d_desc, S->d_work, S->nrow, row_grid(S), fused_tw(S->mesh),
-                          stream_of(S->mesh));
+                          stream_of(S->mesh), fin);
 }
 int col_stage_raw(qs_solver* S, int mode) {
   QS_TRY(fused_check(S));
@@ -131,7 +132,7 @@ int timed(qs_solver* S, int cls, F launch) {
   return QS_OK;
 }
 
-int row_rhs(qs_solver* S) { return timed(S, K_ROW, [&] { return row_rhs_raw(S); }); }
+int row_rhs(qs_solver* S, int fin = 0) { return timed(S, K_ROW, [&] { return row_rhs_raw(S, fin); }); }
 int col_stage(qs_solver* S, int mode) {
   // intermediate stages (IFRK2 1, RK4 1-3) and final stages (IFRK2 2, RK4 4, IFE)
   const int cls = is_prologue(mode) ? K_COL_PRO
@@ -214,7 +215,7 @@ int qs_solver_create(Mesh* mesh, const qs_params* params, qs_solver** out) {
   for (int k = 0; k < 7 && ok; ++k) ok = hipMalloc((void**)&S->mix[k], hs) == hipSuccess;
   ok = ok && hipMalloc((void**)&S->w00, sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void**)&S->flag, sizeof(int)) == hipSuccess;
-  ok = ok && hipMalloc((void**)&S->ify, sizeof(double) * 4 * (size_t)mesh->Ny) == hipSuccess;
+  ok = ok && hipMalloc((void**)&S->ify, sizeof(double) * ifac_len(mesh->Nxh, mesh->Ny)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&S->hflag, sizeof(int), hipHostMallocDefault) == hipSuccess;
   if (S->hflag) *S->hflag = 0;
   ok = ok && hipMalloc((void**)&S->d_desc, sizeof(RunDesc)) == hipSuccess;
@@ -238,7 +239,8 @@ int qs_solver_create(Mesh* mesh, const qs_params* params, qs_solver** out) {
   if (st == QS_OK && hipMemcpy(S->d_work, work.data(), sizeof(int2) * nwork, hipMemcpyHostToDevice) != hipSuccess)
     st = fail(QS_ERR_HIP, "qs_solver_create: work list");
   if (st == QS_OK)
-    st = ifac_rows_launch(S->ify, mesh->Ny, 2 * M_PI / mesh->Ly, params->Re, params->D_Q, params->dt, sm);
+    st = ifac_launch(S->ify, mesh->Nxh, mesh->Ny, 2 * M_PI / mesh->Lx, 2 * M_PI / mesh->Ly, params->Re,
+                     params->Rf, params->D_Q, params->a_Q, params->dt, sm);
   if (st == QS_OK) st = upload_desc(S);
   if (st == QS_OK && hipStreamSynchronize(sm) != hipSuccess) st = fail(QS_ERR_HIP, "qs_solver_create: sync");
   if (st != QS_OK) {
@@ -336,7 +338,7 @@ int qs_step_ifrk2(qs_solver* S, int nsteps) {
       QS_TRY(col_stage(S, MODE_S1));
     }
     Range r2("ifrk2 stage 2");
-    QS_TRY(row_rhs(S));
+    QS_TRY(row_rhs(S, 1));
     QS_TRY(col_stage(S, MODE_S2));
   }
   return nonfinite_publish(S);
@@ -359,7 +361,7 @@ int qs_step_rk4(qs_solver* S, int nsteps) {
     }
     for (int mode : {MODE_R1, MODE_R2, MODE_R3, MODE_R4}) {
       Range rq("rk4 stage %d", mode - MODE_R1 + 1);
-      QS_TRY(row_rhs(S));
+      QS_TRY(row_rhs(S, mode == MODE_R4));
       QS_TRY(col_stage(S, mode));
     }
   }
@@ -380,7 +382,7 @@ int qs_step_ife(qs_solver* S, int nsteps) {
       QS_TRY(col_stage(S, MODE_PRO_E));
       S->mixed_valid = true;
     }
-    QS_TRY(row_rhs(S));
+    QS_TRY(row_rhs(S, 1));
     QS_TRY(col_stage(S, MODE_E));
   }
   return nonfinite_publish(S);

@@ -47,7 +47,7 @@ struct qs_solver {
   double* w00 = nullptr;                             // W(0,0) of the stage
   int* flag = nullptr;                               // non-finite flag (device)
   int* hflag = nullptr;                              // its host copy (pinned), refreshed after each fused call
-  double* ify = nullptr;                             // [4][Ny] integrating-factor row factors (IFac)
+  double* ify = nullptr;                             // integrating-factor tables (IFac, ifac_len doubles)
   qs::RunDesc* d_desc = nullptr;
   int2* d_work = nullptr;
   int nrow = 0, ncol = 0;
