@@ -156,7 +156,7 @@ def adjoint_bench(reps=3):
             "adjoint_device_bytes": s.adj_memory(), "timing": f"best of {reps}, wall clock between syncs"}
 
 
-def ensemble_bench(nsteps=200, loop_steps=20, tune_steps=3):
+def ensemble_bench(nsteps=200, loop_steps=20, tune_steps=10):
     """64 runs at 128^2, 128 at 256^2, 16 at 512^2 (FP64 IFRK2, dt = 1e-3),
     run r: uniform Q in [-0.01, 0.01] (splitmix64 seed 1000 + r), omega = 0,
     activity ~ U[0.5, 2], friction ~ U[0.01, 0.5] (numpy seed 3).  Aggregate

@@ -299,6 +299,9 @@ typedef struct qs_ensemble_config {
   int lanes[8];        /* elements per lane of the per-wave FFT per size class (its first-pass radix;
                           rows per row job and column-tile width follow), 0 = the library default;
                           sides 64, 128, 256 also take 4 or 8 (256: 4 or 16), 512 also takes 8 */
+  int merged;          /* 1: one persistent kernel per phase over the jobs of every class (16 elements
+                          per lane for every side; lanes and concurrent are then unused, row_blocks[0] /
+                          col_blocks[0] give its block counts, 0 = one per CU / one per job) */
 } qs_ensemble_config;
 int qs_ensemble_create(int n, const int* N, const qs_params* params, qs_ensemble** out);
 int qs_ensemble_destroy(qs_ensemble* e);

@@ -11,7 +11,9 @@
 // classes are independent: with `concurrent` they are parallel branches of the
 // captured graph (fork / join through events), so small classes fill the CUs
 // that large ones leave idle.  The kernels are persistent: a launch of B blocks
-// loops over the work list (B = one per job, or fewer).
+// loops over the work list (B = one per job, or fewer).  With `merged`, one
+// persistent kernel per phase (qs_fused_mix.hip) covers the jobs of every class
+// through a single work list of (run, job) pairs, largest side first.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -51,7 +53,11 @@ struct qs_ensemble {
   hipGraphExec_t exec = nullptr;
   int exec_steps = 0;
   int order_built = -1;
-  int lanes_built[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  int merged_built = -1;
+  int lanes_built[8] = {-1, -1, -1, -1, -1, -1, -1, -1};   // elements per lane of each class's inverse sets
+  int2* d_row_all = nullptr;               // merged work lists (every class)
+  int2* d_col_all = nullptr;
+  int nrow_all = 0, ncol_all = 0, cap_row_all = 0, cap_col_all = 0;
   qs::AdjGroup* adj = nullptr;             // batched adjoint (one mesh size)
 };
 
@@ -66,16 +72,37 @@ int upload_descs(qs_ensemble* E) {
   return QS_OK;
 }
 
-// Work lists of every class in the configured order.
+// Elements per lane that class ci's kernels use under the current config.
+int class_el(const qs_ensemble* E, int ci) {
+  if (E->cfg.merged) return 16;
+  return E->cfg.lanes[ci] ? E->cfg.lanes[ci] : wave_e(E->cls[ci].N);
+}
+
+int upload_list(std::vector<int2>& v, int2** d, int* cap, hipStream_t st) {
+  if ((int)v.size() > *cap) {
+    (void)hipStreamSynchronize(st);
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
+    *cap = 0;
+    if (hipMalloc((void**)d, sizeof(int2) * v.size()) != hipSuccess) return fail(QS_ERR_OOM, "ensemble work lists");
+    *cap = (int)v.size();
+  }
+  QS_HIP(hipMemcpy(*d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice));
+  return QS_OK;
+}
+
+// Work lists of every class in the configured order (and, merged, one list of
+// all classes, largest side first).
 int build_work(qs_ensemble* E) {
-  bool same = E->order_built == E->cfg.order;
-  for (size_t ci = 0; ci < E->cls.size(); ++ci) same = same && E->lanes_built[ci] == E->cfg.lanes[ci];
+  bool same = E->order_built == E->cfg.order && E->merged_built == E->cfg.merged;
+  for (size_t ci = 0; ci < E->cls.size(); ++ci) same = same && E->lanes_built[ci] == class_el(E, (int)ci);
   if (same) return QS_OK;
-  for (size_t ci = 0; ci < E->cls.size(); ++ci) {
+  std::vector<int2> row_all, col_all;
+  for (int ci = (int)E->cls.size() - 1; ci >= 0; --ci) {
     EnsClass& c = E->cls[ci];
     std::vector<int2> row, col;
     const RunDesc d0 = make_desc(E->runs[c.members[0]]);
-    const int el = E->cfg.lanes[ci];
+    const int el = class_el(E, ci);
     const int nr = row_jobs(d0, el), nc = col_jobs(d0, el);
     if (E->cfg.order == 0) {
       for (int r : c.members) {
@@ -88,25 +115,23 @@ int build_work(qs_ensemble* E) {
       for (int j = 0; j < nc; ++j)
         for (int r : c.members) col.push_back(make_int2(r, j));
     }
-    if ((int)row.size() > c.cap_row || (int)col.size() > c.cap_col) {
-      (void)hipStreamSynchronize(E->stream);
-      if (c.d_row) (void)hipFree(c.d_row);
-      if (c.d_col) (void)hipFree(c.d_col);
-      c.d_row = c.d_col = nullptr;
-      if (hipMalloc((void**)&c.d_row, sizeof(int2) * row.size()) != hipSuccess ||
-          hipMalloc((void**)&c.d_col, sizeof(int2) * col.size()) != hipSuccess)
-        return fail(QS_ERR_OOM, "ensemble work lists");
-      c.cap_row = (int)row.size();
-      c.cap_col = (int)col.size();
-    }
+    QS_TRY(upload_list(row, &c.d_row, &c.cap_row, E->stream));
+    QS_TRY(upload_list(col, &c.d_col, &c.cap_col, E->stream));
     if (E->lanes_built[ci] != el) c.valid = false;   // inverse sets from this layout's prologue
-    QS_HIP(hipMemcpy(c.d_row, row.data(), sizeof(int2) * row.size(), hipMemcpyHostToDevice));
-    QS_HIP(hipMemcpy(c.d_col, col.data(), sizeof(int2) * col.size(), hipMemcpyHostToDevice));
     c.nrow = (int)row.size();
     c.ncol = (int)col.size();
     E->lanes_built[ci] = el;
+    row_all.insert(row_all.end(), row.begin(), row.end());
+    col_all.insert(col_all.end(), col.begin(), col.end());
+  }
+  if (E->cfg.merged) {
+    QS_TRY(upload_list(row_all, &E->d_row_all, &E->cap_row_all, E->stream));
+    QS_TRY(upload_list(col_all, &E->d_col_all, &E->cap_col_all, E->stream));
+    E->nrow_all = (int)row_all.size();
+    E->ncol_all = (int)col_all.size();
   }
   E->order_built = E->cfg.order;
+  E->merged_built = E->cfg.merged;
   return QS_OK;
 }
 
@@ -125,10 +150,32 @@ int class_step(qs_ensemble* E, int ci, hipStream_t st) {
   return QS_OK;
 }
 
-// One step of every class on stream `main`: parallel branches (fork / join
-// through events) or all on `main`.  Captures use their own events, so no
-// event is shared between a graph and directly launched work.
+// The merged launches' block counts: row pass one block per CU (its LDS
+// admits one) unless configured, column stages one per job unless configured.
+int merged_row_grid(const qs_ensemble* E) {
+  const int ncu = E->cls[0].mesh->priv->ncu;
+  const int want = E->cfg.row_blocks[0] > 0 ? E->cfg.row_blocks[0] : (ncu > 0 ? ncu : E->nrow_all);
+  return std::min(want, E->nrow_all);
+}
+int merged_col_grid(const qs_ensemble* E) { return blocks(E->cfg.col_blocks[0], E->ncol_all); }
+
+// One step of every class as four launches on `st` (merged).
+int merged_step(qs_ensemble* E, hipStream_t st) {
+  const double2* tw = E->cls[0].mesh->priv->tw1024;
+  const int rb = merged_row_grid(E), cb = merged_col_grid(E);
+  QS_TRY(launch_mix_row(E->d_desc, E->d_row_all, E->nrow_all, rb, tw, st, 0));
+  QS_TRY(launch_mix_col(MODE_S1, E->d_desc, E->d_col_all, E->ncol_all, cb, tw, st));
+  QS_TRY(launch_mix_row(E->d_desc, E->d_row_all, E->nrow_all, rb, tw, st, 1));
+  QS_TRY(launch_mix_col(MODE_S2, E->d_desc, E->d_col_all, E->ncol_all, cb, tw, st));
+  return QS_OK;
+}
+
+// One step of every class on stream `main`: merged launches, parallel
+// branches (fork / join through events) or all on `main`.  Captures use their
+// own events, so no event is shared between a graph and directly launched
+// work.
 int ensemble_step_direct(qs_ensemble* E, hipStream_t main, bool capturing) {
+  if (E->cfg.merged) return merged_step(E, main);
   const int nc = (int)E->cls.size();
   if (!E->cfg.concurrent || nc == 1) {
     for (int ci = 0; ci < nc; ++ci) QS_TRY(class_step(E, ci, main));
@@ -178,6 +225,17 @@ int capture_graph(qs_ensemble* E, int G) {
 }
 
 int prologues(qs_ensemble* E) {
+  if (E->cfg.merged) {
+    // one merged prologue when any class is stale: recomputing a valid class's
+    // inverse sets from its state gives the same values again
+    bool stale = false;
+    for (auto& c : E->cls) stale = stale || !c.valid;
+    if (!stale) return QS_OK;
+    QS_TRY(launch_mix_col(MODE_PRO, E->d_desc, E->d_col_all, E->ncol_all, merged_col_grid(E),
+                          E->cls[0].mesh->priv->tw1024, E->stream));
+    for (auto& c : E->cls) c.valid = true;
+    return QS_OK;
+  }
   for (size_t ci = 0; ci < E->cls.size(); ++ci) {
     EnsClass& c = E->cls[ci];
     if (c.valid) continue;
@@ -207,6 +265,8 @@ void ensemble_free(qs_ensemble* E) {
   drop_graph(E);
   adj_group_free(E->adj);
   for (auto* s : E->runs) qs_solver_destroy(s);
+  if (E->d_row_all) (void)hipFree(E->d_row_all);
+  if (E->d_col_all) (void)hipFree(E->d_col_all);
   for (auto& c : E->cls) {
     if (c.d_row) (void)hipFree(c.d_row);
     if (c.d_col) (void)hipFree(c.d_col);
@@ -342,8 +402,12 @@ int qs_ensemble_get_config(const qs_ensemble* E, qs_ensemble_config* cfg) {
 
 int qs_ensemble_set_config(qs_ensemble* E, const qs_ensemble_config* cfg) {
   if (!E || !cfg) return fail(QS_ERR_INVALID_ARG, "null argument");
-  if (cfg->graph_steps < 0 || (cfg->order != 0 && cfg->order != 1))
-    return fail(QS_ERR_INVALID_ARG, "qs_ensemble_set_config: graph_steps %d order %d", cfg->graph_steps, cfg->order);
+  if (cfg->graph_steps < 0 || (cfg->order != 0 && cfg->order != 1) || (cfg->merged != 0 && cfg->merged != 1))
+    return fail(QS_ERR_INVALID_ARG, "qs_ensemble_set_config: graph_steps %d order %d merged %d", cfg->graph_steps,
+                cfg->order, cfg->merged);
+  if (cfg->merged)
+    for (auto& c : E->cls)
+      if (!mix_side(c.N)) return fail(QS_ERR_UNSUPPORTED_SIZE, "qs_ensemble_set_config: merged kernels, side %d", c.N);
   for (size_t ci = 0; ci < E->cls.size(); ++ci) {
     if (cfg->lanes[ci] == 0) continue;
     int el[4];
@@ -359,9 +423,10 @@ int qs_ensemble_set_config(qs_ensemble* E, const qs_ensemble_config* cfg) {
   return build_work(E);
 }
 
-// Coordinate-descent autotuner over the launch configuration: work-list
-// order, branch concurrency, per-class lane layout of the per-wave FFT (its
-// first-pass radix, rows per row job and column-tile width) and block counts,
+// Coordinate-descent autotuner over the launch configuration: merged or
+// per-class kernels, work-list order, branch concurrency, per-class lane
+// layout of the per-wave FFT (its first-pass radix, rows per row job and
+// column-tile width) and block counts (per class, or of the merged kernels),
 // each candidate timed (HIP events on the ensemble's stream) over about
 // `trial_steps` directly launched steps after a warm-up; then the graph depth,
 // each candidate captured and timed through its graph launches.  The states
@@ -415,7 +480,27 @@ int qs_ensemble_tune(qs_ensemble* E, int trial_steps, qs_ensemble_config* best_o
     if (ms < bms) { bms = ms; best = c; }
     return QS_OK;
   };
+  bool can_merge = true;
+  for (auto& c : E->cls) can_merge = can_merge && mix_side(c.N);
   for (int sweep = 0; sweep < 2 && st == QS_OK; ++sweep) {
+    if (can_merge)
+      for (int v : {0, 1}) {
+        qs_ensemble_config c = best; c.merged = v;
+        if (st == QS_OK) st = try_cfg(c);
+      }
+    if (best.merged) {   // one kernel per phase: its order and block counts
+      for (int v : {0, 1}) {
+        qs_ensemble_config c = best; c.order = v;
+        if (st == QS_OK) st = try_cfg(c);
+      }
+      for (int b : {0, 128, 256, 512, 1024, 2048}) {
+        qs_ensemble_config c = best; c.row_blocks[0] = b;
+        if (st == QS_OK) st = try_cfg(c);
+        c = best; c.col_blocks[0] = b;
+        if (st == QS_OK) st = try_cfg(c);
+      }
+      continue;
+    }
     for (int v : {0, 1}) {
       qs_ensemble_config c = best; c.concurrent = v;
       if (st == QS_OK) st = try_cfg(c);

@@ -77,6 +77,16 @@ int launch_fused_col_l(int nfix, int el, int mode, const RunDesc* runs, const in
 // their count (at most 4).
 int lane_layouts(int n, int* el);
 
+// One kernel per phase over runs of every power-of-two side 16..1024 (mix_side)
+// at 16 elements per lane (qs_fused_mix.hip): the ensemble's merged launches.
+// Job counts are row_jobs / col_jobs with el = 16; modes MODE_PRO, MODE_S1,
+// MODE_S2.
+bool mix_side(int n);
+int launch_mix_row(const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw1024,
+                   hipStream_t st, int fin);
+int launch_mix_col(int mode, const RunDesc* runs, const int2* work, int nwork, int grid, const double2* tw1024,
+                   hipStream_t st);
+
 // Discrete adjoint kernels (qs_adjoint.hip).  nfix > 0 only.
 int launch_adj_row(int nfix, const RunDesc* runs, const int2* work, int nwork, int grid,
                    const double2* tw1024, hipStream_t st);

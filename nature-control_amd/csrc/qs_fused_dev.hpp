@@ -12,6 +12,23 @@ namespace {
 constexpr int kBuf = 1088;           // LDS entries of one wave buffer (1024 + 1024/16)
 constexpr int kTwq = 256;            // staged twiddles: the quarter table of W_1024 (or of W_768)
 
+// threadIdx.x as a value the compiler treats as new on every call: lane
+// offsets derived from it are recomputed per job instead of being hoisted out
+// of a persistent job loop and held (or spilled) across it.
+__device__ __forceinline__ int job_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+// x as a value the compiler treats as new at every call: what is derived from
+// it (wavenumbers, integrating factors) is recomputed where it is used instead
+// of being kept live, or spilled, across the transforms of a column job.
+__device__ __forceinline__ int opq(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+
 // W_N^m from the staged quarter table: of W_1024 for N = 2^a <= 1024, of W_768
 // for N = 3 * 2^a <= 768 (the kernel is handed the matching table).
 template <int N>
@@ -218,12 +235,12 @@ struct RowLoader {
 // (vector g of wave buffer q at sm[q*kBuf + g*VS + swz(x)]).  v: scratch
 // registers of the caller; w00: the run's W(0,0) (*a.w00).  Ends with a
 // barrier.
-template <int N, int EL = wave_e(N), class TW>
+template <int N, int EL = wave_e(N), bool OPQ = false, class TW>
 __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, double2* sm, const TW& tw,
                                                double2 (&v)[16], double w00) {
   using L = WaveLay<N, EL>;
   constexpr int NV = L::NV, LPC = L::LPC, VS = L::VS, E = L::E, H = N / 2 + 1, RH = NV * H;
-  const int tid = threadIdx.x;
+  const int tid = OPQ ? job_tid() : (int)threadIdx.x;   // OPQ: see row_body
   // the wave index as a scalar: the branches below are wave-uniform, so each
   // wave runs only its own unrolled loop, with all its LDS reads in flight
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
@@ -288,22 +305,6 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
 // covers whole row segments of the tile, while the FFTs run per wavefront on
 // the lane layout "lane (lp, g) holds rows lp + LPC*m, m < E".  A tile holds
 // TC * NY = 256 * E elements: E per thread.
-// threadIdx.x as a value the compiler treats as new on every call: lane
-// offsets derived from it are recomputed per job instead of being hoisted out
-// of a persistent job loop and held (or spilled) across it.
-__device__ __forceinline__ int job_tid() {
-  int t = threadIdx.x;
-  asm volatile("" : "+v"(t));
-  return t;
-}
-// x as a value the compiler treats as new at every call: what is derived from
-// it (wavenumbers, integrating factors) is recomputed where it is used instead
-// of being kept live, or spilled, across the transforms of a column job.
-__device__ __forceinline__ int opq(int x) {
-  asm volatile("" : "+v"(x));
-  return x;
-}
-
 template <int NY, int EL = wave_e(NY)>
 struct ColTile {
   static constexpr int CW = WaveLay<NY, EL>::NV, TC = 4 * CW, LPC = WaveLay<NY, EL>::LPC, VS = WaveLay<NY, EL>::VS;

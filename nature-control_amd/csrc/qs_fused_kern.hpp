@@ -27,20 +27,24 @@ namespace {
 // Block barriers: kRowBodyBarriers (two in inverse_fields, one after the
 // products, one after the forward FFTs).
 constexpr int kRowBodyBarriers = 4;
-template <int N, int EL = wave_e(N), class TW>
+// OPQ: lane offsets from an opaque thread index (job_tid), recomputed per job:
+// the merged kernels need it (their several sizes' job code in one function
+// spills otherwise); the fixed-size kernels keep the precomputed offsets
+// (faster there).
+template <int N, int EL = wave_e(N), bool OPQ = false, class TW>
 __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm, const TW& tw, double w00,
                                          double alu, int kout) {
   using L = WaveLay<N, EL>;
   constexpr int NV = L::NV;             // rows of this job (= vectors per wave FFT)
   constexpr int LPC = L::LPC, E = L::E;
   constexpr int VS = L::VS;             // padded vector stride inside a wave buffer
-  const int tid = threadIdx.x;
+  const int tid = OPQ ? job_tid() : (int)threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;     // lane = lp * NV + g
   double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
 
   double2 v[16];
-  inverse_fields<N, EL>(a, row0, sm, tw, v, w00);
+  inverse_fields<N, EL, OPQ>(a, row0, sm, tw, v, w00);
 
   // ---- 4. pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) --
   for (int p = tid; p < NV * N; p += 384) {
@@ -123,9 +127,11 @@ __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm
 // ===========================================================================
 // Column job: one field (FLD 0 = w, 1 = r1, 2 = r2) on a tile of 4 NV
 // columns (ColTile), one column per lane group, FFTs along y per wavefront.
+// OPQ: each transform derives its lane offsets from an opaque lane index (for
+// the merged kernels, as in row_body).
 // Every input tile is staged through the wave buffers (ColTile::load) and
 // every output tile leaves through them (ColTile::store).
-template <int NY, int MODE, int EL = wave_e(NY), class TW>
+template <int NY, int MODE, int EL = wave_e(NY), bool OPQ = false, class TW>
 __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0, double2* sm, const TW& tw,
                                         bool pro_arg) {
   // prologue: final-stage kernels only (see is_prologue)
@@ -159,11 +165,11 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     const double2* state_in = from_old ? a.old_[FLD] : a.new_[FLD];   // R2, R3: new first
     const int f0 = FLD == 0 ? 0 : (FLD == 1 ? 3 : 5);
     T.stage_in(a.mix[f0], st);
-    wave_fft<NY, -1, EL>(st, vb, lp, tw);
+    wave_fft<NY, -1, EL>(st, vb, OPQ ? opq(lp) : lp, tw);
     {
       double2 v[16];
       T.stage_in(a.mix[f0 + 1], v);
-      wave_fft<NY, -1, EL>(v, vb, lp, tw);
+      wave_fft<NY, -1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const double ky = kyj(opq(lp) + LPC * m);
@@ -173,7 +179,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     if (FLD == 0) {
       double2 v[16];
       T.stage_in(a.mix[2], v);
-      wave_fft<NY, -1, EL>(v, vb, lp, tw);
+      wave_fft<NY, -1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const double ky = kyj(opq(lp) + LPC * m);
@@ -277,7 +283,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       double2 v[16];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = st[m];
-      wave_fft<NY, +1, EL>(v, vb, lp, tw);
+      wave_fft<NY, +1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
       stage_out(a.mout[0], v);
     }
 #pragma unroll
@@ -291,7 +297,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       double2 v[16];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = st[m];
-      wave_fft<NY, +1, EL>(v, vb, lp, tw);
+      wave_fft<NY, +1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
       stage_out(a.mout[1], v);
     }
 #pragma unroll
@@ -299,7 +305,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const double ky = kyj(opq(lp) + LPC * m);
       st[m] = make_double2(-ky * st[m].y, ky * st[m].x);
     }
-    wave_fft<NY, +1, EL>(st, vb, lp, tw);
+    wave_fft<NY, +1, EL>(st, vb, OPQ ? opq(lp) : lp, tw);
     stage_out(a.mout[2], st);
   } else {
     const int m0 = FLD == 1 ? 3 : 5;
@@ -307,7 +313,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       double2 v[16];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = st[m];
-      wave_fft<NY, +1, EL>(v, vb, lp, tw);
+      wave_fft<NY, +1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
       stage_out(a.mout[m0], v);
     }
 #pragma unroll
@@ -315,7 +321,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const double ky = kyj(opq(lp) + LPC * m);
       st[m] = make_double2(-ky * ky * st[m].x, -ky * ky * st[m].y);
     }
-    wave_fft<NY, +1, EL>(st, vb, lp, tw);
+    wave_fft<NY, +1, EL>(st, vb, OPQ ? opq(lp) : lp, tw);
     stage_out(a.mout[m0 + 1], st);
   }
   if ((MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && bad) atomicOr(a.flag, 1);
@@ -407,7 +413,7 @@ __device__ __forceinline__ void col_zero_job(const RunDesc& a, int fld, int i0) 
   }
 }
 
-template <int NY, int MODE, int EL = wave_e(NY), class TW>
+template <int NY, int MODE, int EL = wave_e(NY), bool OPQ = false, class TW>
 __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const TW& tw, bool pro) {
   constexpr int tc = 4 * WaveLay<NY, EL>::NV;
   const int ntiles = (a.Nxh + tc - 1) / tc;
@@ -423,7 +429,7 @@ __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double
   } else {
     col_job_map(job, ntiles, fld, tile);
   }
-  col_job<NY, MODE, EL>(a, fld, tile * tc, sm, tw, pro);
+  col_job<NY, MODE, EL, OPQ>(a, fld, tile * tc, sm, tw, pro);
 }
 
 // Every run of a column kernel's work list has Ny = NFIX (its y FFTs; the

@@ -541,12 +541,12 @@ def adj_loss(mesh, W, kind=ENSTROPHY):
 class EnsembleConfig(ctypes.Structure):
     """qs_ensemble_config."""
     _fields_ = [("graph_steps", _I), ("concurrent", _I), ("order", _I),
-                ("row_blocks", _I * 8), ("col_blocks", _I * 8), ("lanes", _I * 8)]
+                ("row_blocks", _I * 8), ("col_blocks", _I * 8), ("lanes", _I * 8), ("merged", _I)]
 
     def as_dict(self):
         return {"graph_steps": self.graph_steps, "concurrent": self.concurrent, "order": self.order,
                 "row_blocks": list(self.row_blocks), "col_blocks": list(self.col_blocks),
-                "lanes": list(self.lanes)}
+                "lanes": list(self.lanes), "merged": self.merged}
 
 
 class Ensemble:
