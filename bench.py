@@ -80,6 +80,41 @@ def copy_calibrate(mib):
 FUSED_SOURCES = ("qs_fused.hip", "qs_fused_kern.hpp", "qs_fused_dev.hpp", "qs_fused.hpp", "qs_internal.hpp")
 
 
+def kept_fraction(N):
+    """Fraction of the Nxh = N/2+1 columns that a final column stage
+    transforms: its tiles that start below the dealias cut kcut = (Nxh-1)*2/3+1,
+    i.e. kcut rounded up to the tile width (4096/N columns at 1024, 512, ...;
+    the kernels' col_kept_cols)."""
+    nxh = N // 2 + 1
+    kcut = (nxh - 1) * 2 // 3 + 1
+    T = (3072 if N % 3 == 0 else 4096) // N
+    kept = min(nxh, -(-kcut // T) * T) if T > 0 else nxh
+    return kept / nxh
+
+
+def alg_H_of(N):
+    """Algorithmic HBM bytes per launch of each kernel class of the fused IFRK2
+    step, in units of H = N*(N/2+1)*16 B (f = kept_fraction(N)):
+      row_rhs      (14 + 7 + 7 f) / 2: both row passes read the 7 inverse-set
+                   arrays; the first writes 7 forward-set arrays, the second
+                   (before the final stage) only their kept columns;
+      col_stage1   20: reads 7 forward-set arrays + 3 state spectra, writes 3
+                   partial sums + 7 inverse-set arrays;
+      col_stage2   10 f + 10: reads the kept columns of 7 forward-set arrays
+                   and 3 partial sums, writes 3 state spectra + 7 inverse-set
+                   arrays (zeros on the masked columns);
+      col_prologue 10: reads 3 state spectra, writes 7 inverse-set arrays."""
+    f = kept_fraction(N)
+    return {"row_rhs": (14 + 7 + 7 * f) / 2, "col_stage1": 20.0, "col_stage2": 10 * f + 10, "col_prologue": 10.0}
+
+
+def step_H_of(N):
+    """Algorithmic bytes of one IFRK2 step (two row passes and two column
+    stages) in units of H."""
+    a = alg_H_of(N)
+    return 2 * a["row_rhs"] + a["col_stage1"] + a["col_stage2"]
+
+
 def traffic_from_profile(kernel, N):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (profiles/fused_kernels_<N>_traffic.md: rocprofv3 FETCH_SIZE and WRITE_SIZE
@@ -217,12 +252,13 @@ def ensemble_bench(nsteps=200, loop_steps=20, tune_steps=10):
     ens_rate = len(sizes) * nsteps / te
     loop_rate = len(sizes) * loop_steps / tl
     H = sum(N * (N // 2 + 1) * 16 for N in sizes)
+    HS = sum(step_H_of(N) * N * (N // 2 + 1) * 16 for N in sizes)   # algorithmic bytes per ensemble step
     e.close()
     return {"workload": "64x128^2 + 128x256^2 + 16x512^2 FP64 IFRK2, dt=1e-3, per-run activity U[0.5,2], "
                         "friction U[0.01,0.5], uniform Q init (seed 1000+r)",
             "run_steps_per_s": round(ens_rate, 1), "ensemble_steps_per_s": round(nsteps / te, 2),
             "loop_run_steps_per_s": round(loop_rate, 1), "speedup_vs_loop": round(ens_rate / loop_rate, 2),
-            "alg_GBps_68H": round(68 * H * nsteps / te / 1e9, 1),
+            "alg_GBps": round(HS * nsteps / te / 1e9, 1),
             "canonical_101H_frac": round(101 * H * nsteps / te / 1e9 / HBM_PEAK_GBS, 4),
             "tuned_config": cfg.as_dict(), "tune_seconds": round(tune_s, 2),
             "timing": f"{nsteps} ensemble steps vs {loop_steps} loop steps, wall clock between syncs"}
@@ -549,17 +585,15 @@ def main():
     finite = not solver.nonfinite()
 
     steps_per_s = args.steps / (ms / 1e3)
-    # Algorithmic HBM bytes per launch of each kernel class (units of H, the
-    # half-spectrum size N*(N/2+1)*16 B): row pass reads 7 and writes 7 mixed
-    # arrays; each column stage reads 7 mixed arrays + 3 state spectra and
-    # writes 3 state spectra + 7 mixed arrays.
-    alg_H = {"row_rhs": 14, "col_stage1": 20, "col_stage2": 20, "col_prologue": 10}
+    # Algorithmic HBM bytes per launch of each kernel class (alg_H_of, units of
+    # H, the half-spectrum size N*(N/2+1)*16 B).
+    alg_H = alg_H_of(N)
     kernels = {}
     for k, (avg_ms, cnt) in prof.items():
         if cnt:
             gbs = alg_H[k] * H / (avg_ms * 1e-3) / 1e9
             kernels[k] = {"avg_us": round(avg_ms * 1e3, 2), "launches": cnt,
-                          "alg_bytes": alg_H[k] * H, "GBps": round(gbs, 1)}
+                          "alg_bytes": round(alg_H[k] * H), "GBps": round(gbs, 1)}
     dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"]) if kernels else None
     roofline = None
     if dom:
@@ -567,7 +601,7 @@ def main():
         roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profile(dom, N)}
-    step_bytes = 68 * H
+    step_bytes = round(step_H_of(N) * H)
     # Counter traffic per step (rocprofv3 FETCH_SIZE + WRITE_SIZE of each class,
     # from the committed profile of this kernel source): two row passes, the two
     # column stages, and the restart prologue once per restart_every steps.
