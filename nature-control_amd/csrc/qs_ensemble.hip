@@ -468,10 +468,13 @@ int qs_ensemble_tune(qs_ensemble* E, int trial_steps, qs_ensemble_config* best_o
     *ms = t / n;
     return QS_OK;
   };
+  // Two descents, one over the per-class kernels and one over the merged
+  // kernels, each from its own start; the faster result wins.  (A single
+  // descent that switched between the two early was led by the noise of short
+  // trials into the other family's untuned corner.)
   qs_ensemble_config best = E->cfg;
   best.graph_steps = 0;                                       // launch shapes first, directly launched
   double bms = 0.0;
-  if (st == QS_OK) st = measure(best, &bms);
   const int nc = (int)E->cls.size();
   auto try_cfg = [&](const qs_ensemble_config& c) -> int {
     double ms = 0.0;
@@ -480,49 +483,54 @@ int qs_ensemble_tune(qs_ensemble* E, int trial_steps, qs_ensemble_config* best_o
     if (ms < bms) { bms = ms; best = c; }
     return QS_OK;
   };
-  bool can_merge = true;
-  for (auto& c : E->cls) can_merge = can_merge && mix_side(c.N);
-  for (int sweep = 0; sweep < 2 && st == QS_OK; ++sweep) {
-    if (can_merge)
-      for (int v : {0, 1}) {
-        qs_ensemble_config c = best; c.merged = v;
-        if (st == QS_OK) st = try_cfg(c);
-      }
-    if (best.merged) {   // one kernel per phase: its order and block counts
+  auto descend = [&](int merged) -> int {
+    best.merged = merged;
+    QS_TRY(measure(best, &bms));
+    for (int sweep = 0; sweep < 2; ++sweep) {
       for (int v : {0, 1}) {
         qs_ensemble_config c = best; c.order = v;
-        if (st == QS_OK) st = try_cfg(c);
+        QS_TRY(try_cfg(c));
       }
-      for (int b : {0, 128, 256, 512, 1024, 2048}) {
-        qs_ensemble_config c = best; c.row_blocks[0] = b;
-        if (st == QS_OK) st = try_cfg(c);
-        c = best; c.col_blocks[0] = b;
-        if (st == QS_OK) st = try_cfg(c);
+      if (merged) {   // one kernel per phase: its block counts
+        for (int b : {0, 128, 256, 512, 1024, 2048}) {
+          qs_ensemble_config c = best; c.row_blocks[0] = b;
+          QS_TRY(try_cfg(c));
+          c = best; c.col_blocks[0] = b;
+          QS_TRY(try_cfg(c));
+        }
+        continue;
       }
-      continue;
-    }
-    for (int v : {0, 1}) {
-      qs_ensemble_config c = best; c.concurrent = v;
-      if (st == QS_OK) st = try_cfg(c);
-    }
-    for (int v : {0, 1}) {
-      qs_ensemble_config c = best; c.order = v;
-      if (st == QS_OK) st = try_cfg(c);
-    }
-    for (int ci = 0; ci < nc && st == QS_OK; ++ci) {
-      int el[4];
-      const int nl = lane_layouts(E->cls[ci].N, el);
-      for (int k = 0; k < nl && st == QS_OK; ++k) {
-        qs_ensemble_config c = best; c.lanes[ci] = el[k];
-        st = try_cfg(c);
+      for (int v : {0, 1}) {
+        qs_ensemble_config c = best; c.concurrent = v;
+        QS_TRY(try_cfg(c));
       }
-      for (int b : {0, 128, 256, 512, 1024, 2048}) {
-        qs_ensemble_config c = best; c.row_blocks[ci] = b;
-        if (st == QS_OK) st = try_cfg(c);
-        c = best; c.col_blocks[ci] = b;
-        if (st == QS_OK) st = try_cfg(c);
+      for (int ci = 0; ci < nc; ++ci) {
+        int el[4];
+        const int nl = lane_layouts(E->cls[ci].N, el);
+        for (int k = 0; k < nl; ++k) {
+          qs_ensemble_config c = best; c.lanes[ci] = el[k];
+          QS_TRY(try_cfg(c));
+        }
+        for (int b : {0, 128, 256, 512, 1024, 2048}) {
+          qs_ensemble_config c = best; c.row_blocks[ci] = b;
+          QS_TRY(try_cfg(c));
+          c = best; c.col_blocks[ci] = b;
+          QS_TRY(try_cfg(c));
+        }
       }
     }
+    return QS_OK;
+  };
+  bool can_merge = true;
+  for (auto& c : E->cls) can_merge = can_merge && mix_side(c.N);
+  const qs_ensemble_config start = best;
+  if (st == QS_OK) st = descend(0);
+  if (st == QS_OK && can_merge) {
+    const qs_ensemble_config b0 = best;
+    const double ms0 = bms;
+    best = start;
+    st = descend(1);
+    if (st == QS_OK && ms0 <= bms) { best = b0; bms = ms0; }
   }
   // graph depth, each candidate captured and launched as a graph
   for (int G : {5, 10, 20}) {
