@@ -54,7 +54,11 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
   using L = WaveLay<N>;
   constexpr int NV = L::NV, LPC = L::LPC, VS = L::VS, E = L::E, H = N / 2 + 1, RH = NV * H;
   const TwN<N> tw{twq};
-  const int tid = threadIdx.x;
+  // at 512 (two rows per job) the lane offsets, held across the job, spilled
+  // (236 B of scratch per lane): there they are recomputed per phase from an
+  // opaque thread index (job_tid); the other sides keep them (faster)
+  constexpr bool OPQ = N == 512;
+  const int tid = OPQ ? job_tid() : (int)threadIdx.x;
   // the wave index as a scalar: wave-uniform branches, one unrolled loop each
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;
@@ -126,7 +130,7 @@ __device__ __forceinline__ void arow_job(const RunDesc& a, int row0, double2* sm
 
   // ---- B. the stage's physical fields (buffers 0-4) -------------------------
   stage_rows<N>(a.minp, a, row0, sm);   // its barrier also publishes the scratch writes
-  inverse_fields<N>(a, row0, sm, tw, v, gload(a.w00));
+  inverse_fields<N, wave_e(N), OPQ>(a, row0, sm, tw, v, gload(a.w00));
 
   // ---- C. pointwise Jacobian transposed (friction/Qsolver.cuh:63-300) -------
   const double alu = a.alpha_p ? gload(a.alpha_p) : a.alpha;
@@ -257,6 +261,9 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
   const TwN<NY> tw{twq};
   ColTile<NY> T(a, sm, i0);
   const int lp = T.lp;
+  // sides >= 512: wavenumbers and LDS slots from an opaque lane index,
+  // recomputed per use (held across the job they spilled); 256 keeps them
+  auto lpo = [&]() { return NY >= 512 ? opq(lp) : lp; };
   const int wave = T.tid >> 6;
   const int i = i0 + wave * ColTile<NY>::CW + T.g;
   const bool iok = i < a.Nxh;
@@ -288,7 +295,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     T.load(a.lam_[FLD]);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int j = lp + LPC * m;
+      const int j = lpo() + LPC * m;
       double2* sl = vb + swz(j);
       const double c = keep(j) ? 1.0 : 0.0;
       const double2 lamv = make_double2(sl->x * c, sl->y * c);
@@ -310,7 +317,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
         wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) {   // psi = W / k^2
-          const int j = lp + LPC * m;
+          const int j = lpo() + LPC * m;
           const double ky = kyj(j);
           const double k2 = kx * kx + ky * ky;
           const double ik2 = (i == 0 && j == 0) || !iok ? 0.0 : recip(k2);
@@ -323,7 +330,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
         wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) {   // i ky psi -> conj: -i ky / k^2
-          const int j = lp + LPC * m;
+          const int j = lpo() + LPC * m;
           const double ky = kyj(j);
           const double k2 = kx * kx + ky * ky;
           const double t = (i == 0 && j == 0) || !iok ? 0.0 : ky * recip(k2);
@@ -340,7 +347,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       wave_fft<NY, -1>(v, vb, lp, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const double ky = kyj(lp + LPC * m);
+        const double ky = kyj(lpo() + LPC * m);
         st[m] = make_double2(st[m].x - ky * ky * v[m].x, st[m].y - ky * ky * v[m].y);
       }
     }
@@ -351,7 +358,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     if (AMODE == AMODE_B) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const int j = lp + LPC * m;
+        const int j = lpo() + LPC * m;
         double2* sl = vb + swz(j);
         const double2 lamv = *sl;
         const double IFj = IFof(j);
@@ -362,7 +369,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     } else if (AMODE == AMODE_A) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const int j = lp + LPC * m;
+        const int j = lpo() + LPC * m;
         double2* sl = vb + swz(j);
         const double c = keep(j) ? 1.0 : 0.0;
         const double2 acc = *sl;
@@ -374,7 +381,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     } else {   // AMODE_END
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const int j = lp + LPC * m;
+        const int j = lpo() + LPC * m;
         const double2 acc = vb[swz(j)];
         vb[swz(j)] = make_double2(acc.x + st[m].x, acc.y + st[m].y);
       }
@@ -388,7 +395,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     const int nyq_x = a.Nx / 2;
 #pragma unroll
     for (int m = 0; m < E; ++m) {   // real part at the self-conjugate bins
-      const int j = lp + LPC * m;
+      const int j = lpo() + LPC * m;
       if ((i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
     }
     {
@@ -402,7 +409,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
       double2 v[16];
 #pragma unroll
       for (int m = 0; m < E; ++m) {   // -i ky
-        const double ky = kyj(lp + LPC * m);
+        const double ky = kyj(lpo() + LPC * m);
         v[m] = make_double2(ky * st[m].y, -ky * st[m].x);
       }
       wave_fft<NY, +1>(v, vb, lp, tw);
@@ -410,7 +417,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {     // ky^2
-      const double ky = kyj(lp + LPC * m);
+      const double ky = kyj(lpo() + LPC * m);
       st[m] = make_double2(ky * ky * st[m].x, ky * ky * st[m].y);
     }
     wave_fft<NY, +1>(st, vb, lp, tw);
@@ -426,7 +433,7 @@ __device__ __forceinline__ void acol_job(const RunDesc& a, const int FLD, int i0
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {     // -i ky
-      const double ky = kyj(lp + LPC * m);
+      const double ky = kyj(lpo() + LPC * m);
       st[m] = make_double2(ky * st[m].y, -ky * st[m].x);
     }
     wave_fft<NY, +1>(st, vb, lp, tw);
