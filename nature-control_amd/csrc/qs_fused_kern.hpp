@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256, 2) void k_col(const RunDesc* __restrict__ runs
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   for (int w = L; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
-    col_dispatch_n<NFIX, MODE, EL>(runs[it.x], it.y, sm, TwN<NFIX>{twq}, pro != 0);
+    col_dispatch_n<NFIX, MODE, EL, (NFIX == 512)>(runs[it.x], it.y, sm, TwN<NFIX>{twq}, pro != 0);
     __syncthreads();
   }
 }
