@@ -146,6 +146,7 @@ struct IFac {
     fy = a.ify + (FLD == 0 ? 0 : NY);
     fyh = a.ify + (FLD == 0 ? 2 * NY : 3 * NY);
   }
+  __device__ __forceinline__ double row(int j) const { return gload(fy + j); }   // exp(c2 ky_j^2 dt)
   __device__ __forceinline__ double IF(int j) const { return fx * gload(fy + j); }
   __device__ __forceinline__ double IFh(int j) const { return fxh * gload(fyh + j); }
 };
@@ -443,6 +444,61 @@ __device__ __forceinline__ void col_job_map_final(int job, int ntiles, int nkeep
     fld = r / zc;
     tile = nkeep + z0 + r % zc;
   }
+}
+
+// Column job placement over the XCDs of an MI355X (8 XCDs of 32 CUs, two
+// column blocks per CU).  The kernels run one block per job; xcd_remap gives
+// XCD x the contiguous job range [J_x, J_x + n_x), J_x = x per + min(x, rem)
+// for n jobs, per = n / 8, rem = n % 8.  Of an XCD's blocks in dispatch order,
+// the first 32 take one CU each and block k >= 32 joins the CU of block
+// k - 32.  A vorticity job (six transforms) sharing a CU runs 20-30 % longer
+// than alone, two r jobs (four each) sharing one end together about when a
+// lone vorticity job does, and a final-stage zero job (stores only) slows its
+// partner little.  So each XCD lists its r jobs first, then its vorticity
+// jobs in the slots that end the first round (no partner), then the rest of
+// its r jobs, the job of the partial tile and the zero jobs, which take the
+// second round beside the r jobs.
+// Applies when the full tiles (nfull = Nxh / tc) split evenly over the XCDs,
+// with at most one partial tile (every fused size from 256 up); returns false
+// otherwise.  nk: full tiles that start below the dealias cut in a final stage
+// (nfull in stage 1); the partial tile, the last, is a zero tile when the
+// caller's kept count ends before it.
+constexpr int kCusPerXcd = 32;
+__host__ __device__ __forceinline__ bool col_job_place(int job, int nxh, int tc, int nk, int& fld, int& tile) {
+  const int nfull = nxh / tc, hasp = nxh % tc != 0;
+  if (nfull < 8 || nfull % 8 != 0 || 3 * (nfull / 8) + 1 > 2 * kCusPerXcd) return false;
+  const int g = nfull / 8, n = 3 * (nfull + hasp), per = n / 8, rem = n % 8;   // per = 3 g
+  const int x = job < rem * (per + 1) ? job / (per + 1) : rem + (job - rem * (per + 1)) / per;
+  const int k = job - (x * per + (x < rem ? x : rem));
+  const int e = x < rem ? 1 : 0;                      // this XCD's job of the partial tile
+  const int nx = 3 * g + e;
+  const int K0 = (x * nk) / 8, hk = ((x + 1) * nk) / 8 - K0;   // kept full tiles K0 .. K0+hk-1
+  const int Z0 = nk + x * g - K0;                     // zero full tiles Z0 .. Z0+g-hk-1
+  // slot of the first vorticity job: the end of the first round, or the front
+  // when every job has a CU of its own
+  const int w0 = nx > kCusPerXcd ? kCusPerXcd - hk : 0;
+  if (k >= w0 && k < w0 + hk) {
+    fld = 0;
+    tile = K0 + (k - w0);
+    return true;
+  }
+  int o = k < w0 ? k : k - hk;                        // rank among the other jobs
+  if (o < 2 * hk) {                                   // r1, then r2, of the kept tiles
+    fld = 1 + o / hk;
+    tile = K0 + o % hk;
+    return true;
+  }
+  o -= 2 * hk;
+  if (o < e) {                                        // the partial tile: field x
+    fld = x;
+    tile = nfull;
+    return true;
+  }
+  o -= e;                                             // zero jobs: 3 per zero full tile
+  const int zc = g - hk > 0 ? g - hk : 1;             // (>= 1 whenever a zero job exists)
+  fld = o / zc;
+  tile = Z0 + o % zc;
+  return true;
 }
 
 }  // namespace

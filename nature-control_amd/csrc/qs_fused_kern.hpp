@@ -143,6 +143,16 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   const bool iok = i < a.Nxh;
   const int ic = iok ? i : 0;
   double2* vb = T.vb;
+  // a wave whose columns all lie beyond Nxh (the partial last tile) takes part
+  // in the tile loads and stores only: its transforms are skipped (their
+  // results are never stored)
+  const bool widle = __builtin_amdgcn_readfirstlane(i0 + (T.tid >> 6) * ColTile<NY, EL>::CW) >= a.Nxh;
+  auto fwd = [&](double2 (&x)[16]) {
+    if (!widle) wave_fft<NY, -1, EL>(x, vb, OPQ ? opq(lp) : lp, tw);
+  };
+  auto inv = [&](double2 (&x)[16]) {
+    if (!widle) wave_fft<NY, +1, EL>(x, vb, OPQ ? opq(lp) : lp, tw);
+  };
   const double kx = a.dkx * ic;
   // ky(j) of the mesh tables (Basic/Mesh.cuh:6-21)
   auto kyj = [&](int j) { return a.dky * (j < NY / 2 + 1 ? j : j - NY); };
@@ -165,11 +175,11 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     const double2* state_in = from_old ? a.old_[FLD] : a.new_[FLD];   // R2, R3: new first
     const int f0 = FLD == 0 ? 0 : (FLD == 1 ? 3 : 5);
     T.stage_in(a.mix[f0], st);
-    wave_fft<NY, -1, EL>(st, vb, OPQ ? opq(lp) : lp, tw);
+    fwd(st);
     {
       double2 v[16];
       T.stage_in(a.mix[f0 + 1], v);
-      wave_fft<NY, -1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
+      fwd(v);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const double ky = kyj(opq(lp) + LPC * m);
@@ -179,7 +189,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
     if (FLD == 0) {
       double2 v[16];
       T.stage_in(a.mix[2], v);
-      wave_fft<NY, -1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
+      fwd(v);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const double ky = kyj(opq(lp) + LPC * m);
@@ -197,6 +207,14 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const int j = opq(lp) + LPC * m;
       if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
     }
+    // the row factors of exp(L dt) for S1 / E, fetched with the state tile (one
+    // memory round trip for both instead of a second one in the update)
+    constexpr bool PRE_IF = MODE == MODE_S1 || MODE == MODE_E;
+    double fyr[16];
+    if (PRE_IF) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) fyr[m] = ifc.row(opq(lp) + LPC * m);
+    }
     // the state tile, staged in the wave buffers and updated in place
     T.load(state_in);
     if (MODE == MODE_S1 || MODE == MODE_S2 || MODE == MODE_R1 || MODE == MODE_R4 || MODE == MODE_E) {
@@ -209,7 +227,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
         const double2 n = st[m];
         const double2 an = make_double2(n.x * a.dt, n.y * a.dt);
         if (MODE == MODE_S1) {
-          const double IF = ifc.IF(j);
+          const double IF = ifc.fx * fyr[m];
           const double2 old = in;
           const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // IFRK2.cuh:16-26
           const double h = 1.0 / 2.0 * IF;
@@ -223,7 +241,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
           *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // RK4.cuh:25-36
           st[m] = make_double2((old.x + an.x / 2.0) * IFh, (old.y + an.y / 2.0) * IFh);
         } else if (MODE == MODE_E) {
-          const double IF = ifc.IF(j);
+          const double IF = ifc.fx * fyr[m];
           const double2 old = in;                                                // IFE.cuh:15-25
           double2 r = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);
           const double cut = keep(j) ? 1.0 : 0.0;
@@ -283,21 +301,21 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       double2 v[16];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = st[m];
-      wave_fft<NY, +1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
+      inv(v);
       stage_out(a.mout[0], v);
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {   // st <- psi
       const int j = opq(lp) + LPC * m;
       const double ky = kyj(j);
-      const double k2 = kx * kx + ky * ky;
-      st[m] = (i == 0 && j == 0) || !iok ? make_double2(0.0, 0.0) : make_double2(st[m].x / k2, st[m].y / k2);
+      const double r = recip(kx * kx + ky * ky);   // one reciprocal for both parts (k^2 = 0: not selected)
+      st[m] = (i == 0 && j == 0) || !iok ? make_double2(0.0, 0.0) : make_double2(st[m].x * r, st[m].y * r);
     }
     {
       double2 v[16];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = st[m];
-      wave_fft<NY, +1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
+      inv(v);
       stage_out(a.mout[1], v);
     }
 #pragma unroll
@@ -305,7 +323,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const double ky = kyj(opq(lp) + LPC * m);
       st[m] = make_double2(-ky * st[m].y, ky * st[m].x);
     }
-    wave_fft<NY, +1, EL>(st, vb, OPQ ? opq(lp) : lp, tw);
+    inv(st);
     stage_out(a.mout[2], st);
   } else {
     const int m0 = FLD == 1 ? 3 : 5;
@@ -313,7 +331,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       double2 v[16];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = st[m];
-      wave_fft<NY, +1, EL>(v, vb, OPQ ? opq(lp) : lp, tw);
+      inv(v);
       stage_out(a.mout[m0], v);
     }
 #pragma unroll
@@ -321,7 +339,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
       const double ky = kyj(opq(lp) + LPC * m);
       st[m] = make_double2(-ky * ky * st[m].x, -ky * ky * st[m].y);
     }
-    wave_fft<NY, +1, EL>(st, vb, OPQ ? opq(lp) : lp, tw);
+    inv(st);
     stage_out(a.mout[m0 + 1], st);
   }
   if ((MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && bad) atomicOr(a.flag, 1);
@@ -418,16 +436,18 @@ __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double
   constexpr int tc = 4 * WaveLay<NY, EL>::NV;
   const int ntiles = (a.Nxh + tc - 1) / tc;
   int fld, tile;
+  const int nfull = a.Nxh / tc;
   if ((MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && !pro) {
     const int nk = (col_kcut(a) + tc - 1) / tc;
     const int nkeep = nk < ntiles ? nk : ntiles;
-    col_job_map_final(job, ntiles, nkeep, fld, tile);
+    if (!col_job_place(job, a.Nxh, tc, nkeep < nfull ? nkeep : nfull, fld, tile))
+      col_job_map_final(job, ntiles, nkeep, fld, tile);
     if (tile >= nkeep) {
       col_zero_job<NY, EL>(a, fld, tile * tc);
       return;
     }
   } else {
-    col_job_map(job, ntiles, fld, tile);
+    if (!col_job_place(job, a.Nxh, tc, nfull, fld, tile)) col_job_map(job, ntiles, fld, tile);
   }
   col_job<NY, MODE, EL, OPQ>(a, fld, tile * tc, sm, tw, pro);
 }
