@@ -145,8 +145,11 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   double2* vb = T.vb;
   // a wave whose columns all lie beyond Nxh (the partial last tile) takes part
   // in the tile loads and stores only: its transforms are skipped (their
-  // results are never stored)
-  const bool widle = __builtin_amdgcn_readfirstlane(i0 + (T.tid >> 6) * ColTile<NY, EL>::CW) >= a.Nxh;
+  // results are never stored).  (Stage-1 modes with one column per wave run
+  // such tiles as col_job_split.)
+  constexpr bool SPLIT1 = (MODE == MODE_S1 || MODE == MODE_R1) && WaveLay<NY, EL>::NV == 1;
+  const bool widle =
+      !SPLIT1 && __builtin_amdgcn_readfirstlane(i0 + (T.tid >> 6) * ColTile<NY, EL>::CW) >= a.Nxh;
   auto fwd = [&](double2 (&x)[16]) {
     if (!widle) wave_fft<NY, -1, EL>(x, vb, OPQ ? opq(lp) : lp, tw);
   };
@@ -407,6 +410,141 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
   }
 }
 
+// Job of stage 1 (IFRK2 S1, RK4 R1) on a partial last tile with one valid
+// column i (every side with one vector per wave: Nxh = nfull * tc + 1), its
+// transforms spread over the block's waves instead of run by one wave with
+// three idle: wave q < nf transforms forward set array f0 + q (vorticity:
+// F0, F1, F2; r: F_a, F_b), wave 0 combines them into N_X, updates the state
+// column and hands the next stage's state to the others through the idle
+// wave's buffer, and wave q < nf then forms and transforms inverse set array
+// m0 + q.  The job's chain is two transforms instead of six (four), so that
+// it no longer outlasts the full tiles' jobs.  Columns move between HBM and
+// registers directly (one column, 16 B per row).  Arithmetic per element as
+// in col_job.
+template <int NY, int MODE, int EL, class TW>
+__device__ __forceinline__ void col_job_split(const RunDesc& a, const int FLD, int i, double2* sm, const TW& tw) {
+  static_assert(MODE == MODE_S1 || MODE == MODE_R1, "stage-1 modes");
+  using L = WaveLay<NY, EL>;
+  static_assert(L::NV == 1, "one column per wave");
+  constexpr int LPC = L::LPC, E = L::E, WS = ColTile<NY, EL>::WS;
+  const int tid = job_tid();
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lp = tid & 63;
+  double2* vb = sm + wave * WS;
+  double2* vst = sm + 3 * WS;   // the fourth wave's buffer: the updated state
+  const int nf = FLD == 0 ? 3 : 2;
+  const int f0 = FLD == 0 ? 0 : (FLD == 1 ? 3 : 5);
+  const double kx = a.dkx * i;
+  auto kyj = [&](int j) { return a.dky * (j < NY / 2 + 1 ? j : j - NY); };
+  auto col_ld = [&](const double2* base, double2 (&v)[16]) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = gload(base + i + (unsigned)((opq(lp) + LPC * m) * a.pitch));
+  };
+  auto col_st = [&](double2* base, const double2 (&v)[16]) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) gstore(base + i + (unsigned)((opq(lp) + LPC * m) * a.pitch), v[m]);
+  };
+  // ---- forward transforms, one array per wave --------------------------------
+  double2 st[16];
+  if (wave < nf) {
+    col_ld(a.mix[f0 + wave], st);
+    wave_fft<NY, -1, EL>(st, vb, lp, tw);
+    if (wave > 0) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) vb[swz(opq(lp) + LPC * m)] = st[m];
+    }
+  }
+  __syncthreads();
+  if (wave == 0) {
+    // N_w = F0 + i ky F1 + ky^2 F2,  N_r = F_a + i ky F_b
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double2 v = sm[WS + swz(opq(lp) + LPC * m)];
+      const double ky = kyj(opq(lp) + LPC * m);
+      st[m] = make_double2(st[m].x - ky * v.y, st[m].y + ky * v.x);
+    }
+    if (FLD == 0) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double2 v = sm[2 * WS + swz(opq(lp) + LPC * m)];
+        const double ky = kyj(opq(lp) + LPC * m);
+        st[m] = make_double2(st[m].x + ky * ky * v.x, st[m].y + ky * ky * v.y);
+      }
+    }
+    const IFac ifc(a, FLD, i, NY);
+    const int nyq_x = a.Nx / 2;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {   // real part of N_w at the self-conjugate bins
+      const int j = opq(lp) + LPC * m;
+      if (FLD == 0 && (i == 0 || i == nyq_x) && (j == 0 || 2 * j == NY)) st[m].y = 0.0;
+    }
+    double2 in[16];
+    col_ld(a.old_[FLD], in);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int j = opq(lp) + LPC * m;
+      const double2 n = st[m];
+      const double2 an = make_double2(n.x * a.dt, n.y * a.dt);
+      const double2 old = in[m];
+      if (MODE == MODE_S1) {
+        const double IF = ifc.IF(j);
+        const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // IFRK2.cuh:16-26
+        const double h = 1.0 / 2.0 * IF;
+        in[m] = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);              // IFRK2.cuh:28-39
+        st[m] = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);      // curr (IFh = IF)
+      } else {
+        const double IF = ifc.IF(j), IFh = ifc.IFh(j);
+        const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // RK4.cuh:12-22
+        const double h = 1.0 / 6.0 * IF;
+        in[m] = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);              // RK4.cuh:25-36
+        st[m] = make_double2((old.x + an.x / 2.0) * IFh, (old.y + an.y / 2.0) * IFh);
+      }
+      vst[swz(j)] = st[m];
+    }
+    col_st(a.new_[FLD], in);
+  }
+  __syncthreads();
+  // ---- inverse set of the next stage, one array per wave ----------------------
+  if (wave < nf) {
+    if (wave > 0) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) st[m] = vst[swz(opq(lp) + LPC * m)];
+    }
+    if (FLD == 0) {
+      // M0 = W, M1 = psi = W/k^2, M2 = i ky psi   (vel_func, QFldfuncs.cuh:204-224)
+      if (wave == 0 && i == 0 && lp == 0 && a.w00o) *(QS_GLOBAL double*)a.w00o = st[0].x;
+      if (wave > 0) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {   // st <- psi
+          const int j = opq(lp) + LPC * m;
+          const double ky = kyj(j);
+          const double r = recip(kx * kx + ky * ky);
+          st[m] = (i == 0 && j == 0) ? make_double2(0.0, 0.0) : make_double2(st[m].x * r, st[m].y * r);
+        }
+      }
+      if (wave == 2) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {   // st <- i ky psi
+          const double ky = kyj(opq(lp) + LPC * m);
+          st[m] = make_double2(-ky * st[m].y, ky * st[m].x);
+        }
+      }
+      wave_fft<NY, +1, EL>(st, vb, lp, tw);
+      col_st(a.mout[wave], st);
+    } else {
+      const int m0 = FLD == 1 ? 3 : 5;
+      if (wave == 1) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {   // st <- -ky^2 R
+          const double ky = kyj(opq(lp) + LPC * m);
+          st[m] = make_double2(-ky * ky * st[m].x, -ky * ky * st[m].y);
+        }
+      }
+      wave_fft<NY, +1, EL>(st, vb, lp, tw);
+      col_st(a.mout[m0 + wave], st);
+    }
+  }
+}
+
 // Job of a final stage on a tile that starts at or beyond kcut (every column
 // masked): the state and the field's inverse-set arrays get zeros, which is
 // what the stage computes there from finite values (and all that later
@@ -448,6 +586,12 @@ __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double
     }
   } else {
     if (!col_job_place(job, a.Nxh, tc, nfull, fld, tile)) col_job_map(job, ntiles, fld, tile);
+    if constexpr ((MODE == MODE_S1 || MODE == MODE_R1) && WaveLay<NY, EL>::NV == 1) {
+      if (tile == nfull && a.Nxh == nfull * tc + 1) {   // one valid column
+        col_job_split<NY, MODE, EL>(a, fld, tile * tc, sm, tw);
+        return;
+      }
+    }
   }
   col_job<NY, MODE, EL, OPQ>(a, fld, tile * tc, sm, tw, pro);
 }
