@@ -216,7 +216,11 @@ struct RowLoader {
   }
   // rows into the staging area, (w00, alu) into sc[0..1]
   __device__ __forceinline__ void commit(double2* sm, double* sc, int lt) const {
-    if (lt == 0) {
+    commit_part(sm, sc, lt, 0, 7);
+  }
+  // arrays t0 .. t1-1 only, at stg + t RH ((w00, alu) with the part from t0 = 0)
+  __device__ __forceinline__ void commit_part(double2* stg, double* sc, int lt, int t0, int t1) const {
+    if (t0 == 0 && lt == 0) {
       sc[0] = w00;
       sc[1] = alu;
     }
@@ -224,7 +228,7 @@ struct RowLoader {
     for (int t = 0; t < 7; ++t)
 #pragma unroll
       for (int h = 0; h < S; ++h)
-        if (lt + 128 * h < RH) sm[t * RH + lt + 128 * h] = ld[t][h];
+        if (t >= t0 && t < t1 && lt + 128 * h < RH) stg[t * RH + lt + 128 * h] = ld[t][h];
   }
 };
 
@@ -236,7 +240,9 @@ struct RowLoader {
 // (vector g of wave buffer q at sm[q*kBuf + g*VS + swz(x)]).  v: scratch
 // registers of the caller; w00: the run's W(0,0) (*a.w00).  Ends with a
 // barrier.
-template <int N, int EL = wave_e(N), bool OPQ = false, class TW>
+// OWNSTG: staged arrays 0-5 at sm + 6 kBuf (an area of their own), array 6
+// at sm + 6 RH (in wave buffer 0); else all seven at sm + t RH.
+template <int N, int EL = wave_e(N), bool OPQ = false, bool OWNSTG = false, class TW>
 __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, double2* sm, const TW& tw,
                                                double2 (&v)[16], double w00) {
   using L = WaveLay<N, EL>;
@@ -247,7 +253,13 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int lp = lane / NV, g = lane % NV;
   double2* vb = sm + wave * kBuf + g * VS;
-  auto arr = [&](int t) -> const double2* { return sm + t * RH + g * H; };   // staged array t, vector g
+  // (the own area's base as a per-job value: beyond the 64 KB reach of an LDS
+  // instruction's offset field, its constant addresses would be kept, and
+  // spilled, across the job loop)
+  const double2* stg = sm + (OWNSTG ? opq(6 * kBuf) : 0);
+  auto arr = [&](int t) -> const double2* {   // staged array t, vector g
+    return (OWNSTG && t < 6 ? stg : sm) + t * RH + g * H;
+  };
   auto build = [&](auto&& pair) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {

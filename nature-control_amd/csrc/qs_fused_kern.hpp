@@ -31,7 +31,8 @@ constexpr int kRowBodyBarriers = 4;
 // the merged kernels need it (their several sizes' job code in one function
 // spills otherwise); the fixed-size kernels keep the precomputed offsets
 // (faster there).
-template <int N, int EL = wave_e(N), bool OPQ = false, class TW>
+// OWNSTG: the staging layout of inverse_fields.
+template <int N, int EL = wave_e(N), bool OPQ = false, bool OWNSTG = false, class TW>
 __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm, const TW& tw, double w00,
                                          double alu, int kout) {
   using L = WaveLay<N, EL>;
@@ -44,7 +45,7 @@ __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm
   double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
 
   double2 v[16];
-  inverse_fields<N, EL, OPQ>(a, row0, sm, tw, v, w00);
+  inverse_fields<N, EL, OPQ, OWNSTG>(a, row0, sm, tw, v, w00);
 
   // ---- 4. pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) --
   for (int p = tid; p < NV * N; p += 384) {
@@ -365,7 +366,12 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
                                                     int nwork, const double2* __restrict__ tw1024, int fin) {
   static_assert(NFIX > 0, "one instantiation per x size");
   constexpr int NT = kRowThreads;
-  __shared__ double2 sm[6 * kBuf];
+  // Staging of 1024-point rows: arrays 0-5 in an area of their own after the
+  // six wave buffers (the LDS holds six of the seven beside the buffers),
+  // array 6 in wave buffer 0.  Other sides: all seven in the wave buffers.
+  constexpr int RH = WaveLay<NFIX, EL>::NV * (NFIX / 2 + 1);
+  constexpr bool OWN = NFIX == 1024 && EL == 16;
+  __shared__ double2 sm[6 * kBuf + (OWN ? 6 * RH : 0)];
   __shared__ double2 twq[kTwq];
   __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
   __shared__ double rsc[2];                  // the job's (w00, alu), from the loader
@@ -386,6 +392,28 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       const int2 it = work[L];
       rl.issue(runs[it.x], it.y * NV, lt);
     }
+    if constexpr (OWN) {
+      // Arrays 0-5 of the next job (and its w00, alu) are committed as soon
+      // as this job's inputs are built (row_body's first barrier), while its
+      // transforms run; only array 6 waits for the job's end.  The compute
+      // waves read (w00, alu) before that barrier.
+      if (L < nwork) rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, 6);
+      for (int w = L; w < nwork; w += gridDim.x) {
+        rl.commit_part(sm, rsc, lt, 6, 7);   // array 6 into wave buffer 0
+        __syncthreads();   // staging ready
+        const int wn = w + gridDim.x;
+        if (wn < nwork) {
+          const int2 it = work[wn];
+          rl.issue(runs[it.x], it.y * NV, lt);
+        }
+        __syncthreads();   // staging area 0-5 free (row_body's first barrier)
+        if (wn < nwork) rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, 6);
+#pragma unroll 1
+        for (int b = 1; b < kRowBodyBarriers; ++b) __syncthreads();
+        __syncthreads();   // job done
+      }
+      return;
+    }
     for (int w = L; w < nwork; w += gridDim.x) {
       rl.commit(sm, rsc, lt);            // this job's rows (waits for the loads)
       __syncthreads();   // staging ready
@@ -403,8 +431,8 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       const int2 it = work[w];          // (run, job)
       __syncthreads();   // staging ready
       const RunDesc& a = runs[it.x];
-      row_body<NFIX, EL>(a, it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1],
-                         fin ? col_kept_cols(a) : NFIX / 2 + 1);
+      row_body<NFIX, EL, false, OWN>(a, it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1],
+                                     fin ? col_kept_cols(a) : NFIX / 2 + 1);
       __syncthreads();   // job done: wave buffers free for the next staging
     }
   }
