@@ -375,6 +375,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
   __shared__ double2 twq[kTwq];
   __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
   __shared__ double rsc[2];                  // the job's (w00, alu), from the loader
+  __shared__ int2 rjob;                      // the job's (run, job), from the loader (OWN)
   stage_twiddles(twq, tw1024, threadIdx.x, NT);
   __syncthreads();
   if constexpr (use_twt<NFIX>()) {
@@ -397,17 +398,25 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       // as this job's inputs are built (row_body's first barrier), while its
       // transforms run; only array 6 waits for the job's end.  The compute
       // waves read (w00, alu) before that barrier.
-      if (L < nwork) rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, 6);
+      // The job's (run, job) goes to the compute waves with (w00, alu).
+      if (L < nwork) {
+        rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, 6);
+        if (lt == 0) rjob = work[L];
+      }
       for (int w = L; w < nwork; w += gridDim.x) {
         rl.commit_part(sm, rsc, lt, 6, 7);   // array 6 into wave buffer 0
         __syncthreads();   // staging ready
         const int wn = w + gridDim.x;
+        int2 itn = make_int2(0, 0);
         if (wn < nwork) {
-          const int2 it = work[wn];
-          rl.issue(runs[it.x], it.y * NV, lt);
+          itn = work[wn];
+          rl.issue(runs[itn.x], itn.y * NV, lt);
         }
         __syncthreads();   // staging area 0-5 free (row_body's first barrier)
-        if (wn < nwork) rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, 6);
+        if (wn < nwork) {
+          rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, 6);
+          if (lt == 0) rjob = itn;
+        }
 #pragma unroll 1
         for (int b = 1; b < kRowBodyBarriers; ++b) __syncthreads();
         __syncthreads();   // job done
@@ -428,8 +437,10 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
     }
   } else {
     for (int w = L; w < nwork; w += gridDim.x) {
-      const int2 it = work[w];          // (run, job)
+      int2 it;
+      if constexpr (!OWN) it = work[w];   // (run, job)
       __syncthreads();   // staging ready
+      if constexpr (OWN) it = rjob;
       const RunDesc& a = runs[it.x];
       row_body<NFIX, EL, false, OWN>(a, it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1],
                                      fin ? col_kept_cols(a) : NFIX / 2 + 1);
