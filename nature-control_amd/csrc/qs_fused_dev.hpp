@@ -240,9 +240,9 @@ struct RowLoader {
 // (vector g of wave buffer q at sm[q*kBuf + g*VS + swz(x)]).  v: scratch
 // registers of the caller; w00: the run's W(0,0) (*a.w00).  Ends with a
 // barrier.
-// OWNSTG: staged arrays 0-5 at sm + 6 kBuf (an area of their own), array 6
-// at sm + 6 RH (in wave buffer 0); else all seven at sm + t RH.
-template <int N, int EL = wave_e(N), bool OPQ = false, bool OWNSTG = false, class TW>
+// NOWN: staged arrays t < NOWN at sm + 6 kBuf + t RH (an area of their own
+// after the six wave buffers), the others at sm + t RH (in the buffers).
+template <int N, int EL = wave_e(N), bool OPQ = false, int NOWN = 0, class TW>
 __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, double2* sm, const TW& tw,
                                                double2 (&v)[16], double w00) {
   using L = WaveLay<N, EL>;
@@ -256,9 +256,9 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
   // (the own area's base as a per-job value: beyond the 64 KB reach of an LDS
   // instruction's offset field, its constant addresses would be kept, and
   // spilled, across the job loop)
-  const double2* stg = sm + (OWNSTG ? opq(6 * kBuf) : 0);
+  const double2* stg = sm + (NOWN > 0 ? opq(6 * kBuf) : 0);
   auto arr = [&](int t) -> const double2* {   // staged array t, vector g
-    return (OWNSTG && t < 6 ? stg : sm) + t * RH + g * H;
+    return (t < NOWN ? stg : sm) + t * RH + g * H;
   };
   auto build = [&](auto&& pair) {
 #pragma unroll

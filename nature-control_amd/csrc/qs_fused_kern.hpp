@@ -31,8 +31,8 @@ constexpr int kRowBodyBarriers = 4;
 // the merged kernels need it (their several sizes' job code in one function
 // spills otherwise); the fixed-size kernels keep the precomputed offsets
 // (faster there).
-// OWNSTG: the staging layout of inverse_fields.
-template <int N, int EL = wave_e(N), bool OPQ = false, bool OWNSTG = false, class TW>
+// NOWN: the staging layout of inverse_fields.
+template <int N, int EL = wave_e(N), bool OPQ = false, int NOWN = 0, class TW>
 __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm, const TW& tw, double w00,
                                          double alu, int kout) {
   using L = WaveLay<N, EL>;
@@ -45,7 +45,7 @@ __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm
   double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
 
   double2 v[16];
-  inverse_fields<N, EL, OPQ, OWNSTG>(a, row0, sm, tw, v, w00);
+  inverse_fields<N, EL, OPQ, NOWN>(a, row0, sm, tw, v, w00);
 
   // ---- 4. pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) --
   for (int p = tid; p < NV * N; p += 384) {
@@ -366,12 +366,14 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
                                                     int nwork, const double2* __restrict__ tw1024, int fin) {
   static_assert(NFIX > 0, "one instantiation per x size");
   constexpr int NT = kRowThreads;
-  // Staging of 1024-point rows: arrays 0-5 in an area of their own after the
-  // six wave buffers (the LDS holds six of the seven beside the buffers),
-  // array 6 in wave buffer 0.  Other sides: all seven in the wave buffers.
+  // Staging: arrays 0 .. NOWN-1 in an area of their own after the six wave
+  // buffers (as many as the 160 KB of LDS holds beside them: six, five for
+  // N = 16), the others in the wave buffers.
   constexpr int RH = WaveLay<NFIX, EL>::NV * (NFIX / 2 + 1);
-  constexpr bool OWN = NFIX == 1024 && EL == 16;
-  __shared__ double2 sm[6 * kBuf + (OWN ? 6 * RH : 0)];
+  constexpr int FREE = 10240 - 6 * kBuf - kTwq - twt_len<NFIX>() - 2;   // 16-byte entries
+  constexpr int NOWN = FREE / RH < 6 ? FREE / RH : 6;
+  static_assert(NOWN >= 1, "own staging area");
+  __shared__ double2 sm[6 * kBuf + NOWN * RH];
   __shared__ double2 twq[kTwq];
   __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
   __shared__ double rsc[2];                  // the job's (w00, alu), from the loader
@@ -393,57 +395,40 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       const int2 it = work[L];
       rl.issue(runs[it.x], it.y * NV, lt);
     }
-    if constexpr (OWN) {
-      // Arrays 0-5 of the next job (and its w00, alu) are committed as soon
-      // as this job's inputs are built (row_body's first barrier), while its
-      // transforms run; only array 6 waits for the job's end.  The compute
-      // waves read (w00, alu) before that barrier.
-      // The job's (run, job) goes to the compute waves with (w00, alu).
-      if (L < nwork) {
-        rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, 6);
-        if (lt == 0) rjob = work[L];
-      }
-      for (int w = L; w < nwork; w += gridDim.x) {
-        rl.commit_part(sm, rsc, lt, 6, 7);   // array 6 into wave buffer 0
-        __syncthreads();   // staging ready
-        const int wn = w + gridDim.x;
-        int2 itn = make_int2(0, 0);
-        if (wn < nwork) {
-          itn = work[wn];
-          rl.issue(runs[itn.x], itn.y * NV, lt);
-        }
-        __syncthreads();   // staging area 0-5 free (row_body's first barrier)
-        if (wn < nwork) {
-          rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, 6);
-          if (lt == 0) rjob = itn;
-        }
-#pragma unroll 1
-        for (int b = 1; b < kRowBodyBarriers; ++b) __syncthreads();
-        __syncthreads();   // job done
-      }
-      return;
+    // Arrays 0 .. NOWN-1 of the next job (with its w00, alu and (run, job))
+    // are committed as soon as this job's inputs are built (row_body's first
+    // barrier), while its transforms run; only the others wait for the job's
+    // end.  The compute waves read what they need of the handoff before that
+    // barrier.  The next job's loads are issued as this one starts.
+    if (L < nwork) {
+      rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, NOWN);
+      if (lt == 0) rjob = work[L];
     }
     for (int w = L; w < nwork; w += gridDim.x) {
-      rl.commit(sm, rsc, lt);            // this job's rows (waits for the loads)
+      rl.commit_part(sm, rsc, lt, NOWN, 7);   // the rest into the wave buffers
       __syncthreads();   // staging ready
-      const int wn = w + gridDim.x;      // the next job's rows, in flight during this one
-      if (wn < nwork) {                  // (issued after the barrier: the compute waves start at once)
-        const int2 it = work[wn];
-        rl.issue(runs[it.x], it.y * NV, lt);
+      const int wn = w + gridDim.x;
+      int2 itn = make_int2(0, 0);
+      if (wn < nwork) {
+        itn = work[wn];
+        rl.issue(runs[itn.x], itn.y * NV, lt);
+      }
+      __syncthreads();   // own staging area free (row_body's first barrier)
+      if (wn < nwork) {
+        rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, NOWN);
+        if (lt == 0) rjob = itn;
       }
 #pragma unroll 1
-      for (int b = 0; b < kRowBodyBarriers; ++b) __syncthreads();
+      for (int b = 1; b < kRowBodyBarriers; ++b) __syncthreads();
       __syncthreads();   // job done
     }
   } else {
     for (int w = L; w < nwork; w += gridDim.x) {
-      int2 it;
-      if constexpr (!OWN) it = work[w];   // (run, job)
       __syncthreads();   // staging ready
-      if constexpr (OWN) it = rjob;
+      const int2 it = rjob;   // (run, job)
       const RunDesc& a = runs[it.x];
-      row_body<NFIX, EL, false, OWN>(a, it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1],
-                                     fin ? col_kept_cols(a) : NFIX / 2 + 1);
+      row_body<NFIX, EL, false, NOWN>(a, it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1],
+                                      fin ? col_kept_cols(a) : NFIX / 2 + 1);
       __syncthreads();   // job done: wave buffers free for the next staging
     }
   }

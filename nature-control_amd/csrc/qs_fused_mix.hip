@@ -42,8 +42,9 @@ struct MixRowLoader {
 #pragma unroll
       for (int h = 0; h < S; ++h) ld[t][h] = gload(a.minp[t] + off[h]);
   }
-  __device__ __forceinline__ void commit(double2* sm, double* sc, int lt) const {
-    if (lt == 0) {
+  // arrays t0 .. t1-1 at stg + t rh ((w00, alu) into sc with the part from t0 = 0)
+  __device__ __forceinline__ void commit_part(double2* stg, double* sc, int lt, int t0, int t1) const {
+    if (t0 == 0 && lt == 0) {
       sc[0] = w00;
       sc[1] = alu;
     }
@@ -51,19 +52,24 @@ struct MixRowLoader {
     for (int t = 0; t < 7; ++t)
 #pragma unroll
       for (int h = 0; h < S; ++h)
-        if (lt + 128 * h < rh) sm[t * rh + lt + 128 * h] = ld[t][h];
+        if (t >= t0 && t < t1 && lt + 128 * h < rh) stg[t * rh + lt + 128 * h] = ld[t][h];
   }
 };
-static_assert(7 * 64 * 9 <= 6 * kBuf, "the staging area of the smallest side fits the wave buffers");
+// Staging: arrays 0-4 in an area of their own after the six wave buffers
+// (committed while the previous job's transforms run), arrays 5-6 in the
+// buffers (committed at the previous job's end).  rh <= 576 (N = 16).
+constexpr int kMixOwn = 5, kMixRh = 64 * 9;
+static_assert(7 * kMixRh <= 6 * kBuf, "the staging area of the smallest side fits the wave buffers");
 
 // Row pass over every class: the block layout of k_row (six compute waves, two
 // loader waves, the same barriers per job).  fin as in k_row.
 __global__ __launch_bounds__(kRowThreads) void k_row_mix(const RunDesc* __restrict__ runs,
                                                         const int2* __restrict__ work, int nwork,
                                                         const double2* __restrict__ tw1024, int fin) {
-  __shared__ double2 sm[6 * kBuf];
+  __shared__ double2 sm[6 * kBuf + kMixOwn * kMixRh];
   __shared__ double2 twq[kTwq];
   __shared__ double rsc[2];
+  __shared__ int2 rjob;   // the job's (run, job), from the loader
   stage_twiddles(twq, tw1024, threadIdx.x, kRowThreads);
   __syncthreads();
   const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -74,30 +80,38 @@ __global__ __launch_bounds__(kRowThreads) void k_row_mix(const RunDesc* __restri
       const int2 it = work[L];
       const RunDesc& a = runs[it.x];
       rl.issue(a, it.y * (64 * kMixEL / a.Nx), lt);
+      rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, kMixOwn);
+      if (lt == 0) rjob = it;
     }
     for (int w = L; w < nwork; w += gridDim.x) {
-      rl.commit(sm, rsc, lt);
+      rl.commit_part(sm, rsc, lt, kMixOwn, 7);
       __syncthreads();   // staging ready
       const int wn = w + gridDim.x;
+      int2 itn = make_int2(0, 0);
       if (wn < nwork) {
-        const int2 it = work[wn];
-        const RunDesc& a = runs[it.x];
-        rl.issue(a, it.y * (64 * kMixEL / a.Nx), lt);
+        itn = work[wn];
+        const RunDesc& a = runs[itn.x];
+        rl.issue(a, itn.y * (64 * kMixEL / a.Nx), lt);
+      }
+      __syncthreads();   // own staging area free (row_body's first barrier)
+      if (wn < nwork) {
+        rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, kMixOwn);
+        if (lt == 0) rjob = itn;
       }
 #pragma unroll 1
-      for (int b = 0; b < kRowBodyBarriers; ++b) __syncthreads();
+      for (int b = 1; b < kRowBodyBarriers; ++b) __syncthreads();
       __syncthreads();   // job done
     }
   } else {
     for (int w = L; w < nwork; w += gridDim.x) {
-      const int2 it = work[w];
       __syncthreads();   // staging ready
+      const int2 it = rjob;
       const RunDesc& a = runs[it.x];
       const int kout = fin ? col_kept_cols(a) : a.Nxh;
       const double w00 = rsc[0], alu = rsc[1];
       switch (a.Nx) {   // uniform across the block
 #define QS_MIX_ROW(NN) \
-        case NN: row_body<NN, kMixEL, true>(a, it.y * (64 * kMixEL / NN), sm, TwN<NN>{twq}, w00, alu, kout); break;
+        case NN: row_body<NN, kMixEL, true, kMixOwn>(a, it.y * (64 * kMixEL / NN), sm, TwN<NN>{twq}, w00, alu, kout); break;
         QS_MIX_ROW(16) QS_MIX_ROW(32) QS_MIX_ROW(64) QS_MIX_ROW(128) QS_MIX_ROW(256) QS_MIX_ROW(512)
         QS_MIX_ROW(1024)
 #undef QS_MIX_ROW
