@@ -147,10 +147,11 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
   // a wave whose columns all lie beyond Nxh (the partial last tile) takes part
   // in the tile loads and stores only: its transforms are skipped (their
   // results are never stored).  (Stage-1 modes with one column per wave run
-  // such tiles as col_job_split.)
+  // such tiles as col_job_split; the kernels with opaque lane indices, which
+  // spill with the branch, run the idle transforms.)
   constexpr bool SPLIT1 = (MODE == MODE_S1 || MODE == MODE_R1) && WaveLay<NY, EL>::NV == 1;
-  const bool widle =
-      !SPLIT1 && __builtin_amdgcn_readfirstlane(i0 + (T.tid >> 6) * ColTile<NY, EL>::CW) >= a.Nxh;
+  const bool widle = !SPLIT1 && !OPQ &&
+                     __builtin_amdgcn_readfirstlane(i0 + (T.tid >> 6) * ColTile<NY, EL>::CW) >= a.Nxh;
   auto fwd = [&](double2 (&x)[16]) {
     if (!widle) wave_fft<NY, -1, EL>(x, vb, OPQ ? opq(lp) : lp, tw);
   };
@@ -231,7 +232,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
         const double2 n = st[m];
         const double2 an = make_double2(n.x * a.dt, n.y * a.dt);
         if (MODE == MODE_S1) {
-          const double IF = ifc.fx * fyr[m];
+          const double IF = PRE_IF ? ifc.fx * fyr[m] : ifc.IF(j);
           const double2 old = in;
           const double2 nw0 = make_double2(old.x * IF, old.y * IF);              // IFRK2.cuh:16-26
           const double h = 1.0 / 2.0 * IF;
@@ -245,7 +246,7 @@ __device__ __forceinline__ void col_job(const RunDesc& a, const int FLD, int i0,
           *sl = make_double2(nw0.x + h * an.x, nw0.y + h * an.y);                // RK4.cuh:25-36
           st[m] = make_double2((old.x + an.x / 2.0) * IFh, (old.y + an.y / 2.0) * IFh);
         } else if (MODE == MODE_E) {
-          const double IF = ifc.fx * fyr[m];
+          const double IF = PRE_IF ? ifc.fx * fyr[m] : ifc.IF(j);
           const double2 old = in;                                                // IFE.cuh:15-25
           double2 r = make_double2((old.x + an.x) * IF, (old.y + an.y) * IF);
           const double cut = keep(j) ? 1.0 : 0.0;
@@ -593,7 +594,10 @@ __device__ __forceinline__ void col_zero_job(const RunDesc& a, int fld, int i0) 
   }
 }
 
-template <int NY, int MODE, int EL = wave_e(NY), bool OPQ = false, class TW>
+// MIX: a job of the merged ensemble kernels, whose work list mixes runs (the
+// XCD placement, made for one run's jobs, and the split partial-tile job are
+// left out: one kernel holds every side's job code).
+template <int NY, int MODE, int EL = wave_e(NY), bool OPQ = false, bool MIX = false, class TW>
 __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double2* sm, const TW& tw, bool pro) {
   constexpr int tc = 4 * WaveLay<NY, EL>::NV;
   const int ntiles = (a.Nxh + tc - 1) / tc;
@@ -602,15 +606,15 @@ __device__ __forceinline__ void col_dispatch_n(const RunDesc& a, int job, double
   if ((MODE == MODE_S2 || MODE == MODE_R4 || MODE == MODE_E) && !pro) {
     const int nk = (col_kcut(a) + tc - 1) / tc;
     const int nkeep = nk < ntiles ? nk : ntiles;
-    if (!col_job_place(job, a.Nxh, tc, nkeep < nfull ? nkeep : nfull, fld, tile))
+    if (MIX || !col_job_place(job, a.Nxh, tc, nkeep < nfull ? nkeep : nfull, fld, tile))
       col_job_map_final(job, ntiles, nkeep, fld, tile);
     if (tile >= nkeep) {
       col_zero_job<NY, EL>(a, fld, tile * tc);
       return;
     }
   } else {
-    if (!col_job_place(job, a.Nxh, tc, nfull, fld, tile)) col_job_map(job, ntiles, fld, tile);
-    if constexpr ((MODE == MODE_S1 || MODE == MODE_R1) && WaveLay<NY, EL>::NV == 1) {
+    if (MIX || !col_job_place(job, a.Nxh, tc, nfull, fld, tile)) col_job_map(job, ntiles, fld, tile);
+    if constexpr (!MIX && (MODE == MODE_S1 || MODE == MODE_R1) && WaveLay<NY, EL>::NV == 1) {
       if (tile == nfull && a.Nxh == nfull * tc + 1) {   // one valid column
         col_job_split<NY, MODE, EL>(a, fld, tile * tc, sm, tw);
         return;

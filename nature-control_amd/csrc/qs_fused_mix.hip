@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256, 2) void k_col_mix(const RunDesc* __restrict__ 
     const RunDesc& a = runs[it.x];
     switch (a.Ny) {   // uniform across the block
 #define QS_MIX_COL(NN) \
-      case NN: col_dispatch_n<NN, MODE, kMixEL, true>(a, it.y, sm, TwN<NN>{twq}, pro != 0); break;
+      case NN: col_dispatch_n<NN, MODE, kMixEL, true, true>(a, it.y, sm, TwN<NN>{twq}, pro != 0); break;
       QS_MIX_COL(16) QS_MIX_COL(32) QS_MIX_COL(64) QS_MIX_COL(128) QS_MIX_COL(256) QS_MIX_COL(512)
       QS_MIX_COL(1024)
 #undef QS_MIX_COL
