@@ -424,12 +424,16 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       __syncthreads();   // job done
     }
   } else {
+    // 1024-point rows: the radix-16 pass factors in registers for the whole kernel
+    const auto tw = [&]() {
+      if constexpr (NFIX == 1024 && EL == 16) return TwN1024P(twq, threadIdx.x & 63);
+      else return fused_tw<NFIX>(twq, twt);
+    }();
     for (int w = L; w < nwork; w += gridDim.x) {
       __syncthreads();   // staging ready
       const int2 it = rjob;   // (run, job)
       const RunDesc& a = runs[it.x];
-      row_body<NFIX, EL, false, NOWN>(a, it.y * NV, sm, fused_tw<NFIX>(twq, twt), rsc[0], rsc[1],
-                                      fin ? col_kept_cols(a) : NFIX / 2 + 1);
+      row_body<NFIX, EL, false, NOWN>(a, it.y * NV, sm, tw, rsc[0], rsc[1], fin ? col_kept_cols(a) : NFIX / 2 + 1);
       __syncthreads();   // job done: wave buffers free for the next staging
     }
   }

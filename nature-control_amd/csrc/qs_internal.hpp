@@ -6,6 +6,7 @@
 
 #include <cstdio>
 #include <string>
+#include <type_traits>
 
 #include "qsolver.h"
 
@@ -484,6 +485,26 @@ __device__ __forceinline__ void pass_load(double2 (&t)[16], const double2* buf, 
   }
 }
 
+// A twiddle accessor with kPre16: it also holds the lane's factors of the
+// radix-16 pass P = 16 of the 1024-point transform (w16[r-1] = w^r, computed
+// once per kernel as below).
+template <class T, class = void>
+struct tw_has_pre16 : std::false_type {};
+template <class T>
+struct tw_has_pre16<T, std::void_t<decltype(T::kPre16)>> : std::integral_constant<bool, T::kPre16> {};
+
+// The powers w^r, r = 1 .. R-1, from w^1, w^2, w^4, w^8 (wp): the power of
+// the lowest set bit times those of the others, in rising order.
+template <int R, int NPOW>
+__device__ __forceinline__ double2 tw_power(const double2 (&wp)[NPOW], int r) {
+  const int lo = __builtin_ctz(r);
+  double2 w = wp[lo];
+#pragma unroll
+  for (int e = lo + 1; e < NPOW; ++e)
+    if (r & (1 << e)) w = cmul(w, wp[e]);
+  return w;
+}
+
 template <int N, int P, int SIGN, int EL = wave_e(N), class TW>
 __device__ __forceinline__ void pass_compute(double2 (&t)[16], int lp, const TW& tw) {
   using S = PassShape<N, P, EL>;
@@ -498,7 +519,15 @@ __device__ __forceinline__ void pass_compute(double2 (&t)[16], int lp, const TW&
     // table and every other w^r is the product of the powers of its set bits
     // (at most three extra products, a few ulp): 4 table reads per butterfly
     // instead of R - 1, and only 4 twiddles live in registers.
-    if constexpr (TW::kTable) {
+    if constexpr (tw_has_pre16<TW>::value && N == 1024 && P == 16 && R == 16) {
+      // precomputed (the same products as below)
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        double2 w = tw.w16[r - 1];
+        if (SIGN > 0) w.y = -w.y;
+        tq[r] = cmul(tq[r], w);
+      }
+    } else if constexpr (TW::kTable) {
       // per-pass table: W^(r k STEP) for r = 1 .. R-1 at tw.pass<P>(k)
       const double2* tp = tw.template pass<P>(k);
 #pragma unroll
@@ -517,14 +546,7 @@ __device__ __forceinline__ void pass_compute(double2 (&t)[16], int lp, const TW&
         wp[e] = w;
       }
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        const int lo = __builtin_ctz(r);
-        double2 w = wp[lo];
-#pragma unroll
-        for (int e = lo + 1; e < NPOW; ++e)
-          if (r & (1 << e)) w = cmul(w, wp[e]);
-        tq[r] = cmul(tq[r], w);
-      }
+      for (int r = 1; r < R; ++r) tq[r] = cmul(tq[r], tw_power<R, NPOW>(wp, r));
     }
     dft_any<R, SIGN>(tq);
   }
