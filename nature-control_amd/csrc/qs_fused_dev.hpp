@@ -41,21 +41,25 @@ struct TwN {
   }
 };
 
-// The 1024-point accessor with the lane's radix-16 pass factors held in
-// registers (kPre16): built once per kernel by the row kernels, whose waves
-// run two such transforms per job, instead of four table reads and eleven
-// products per transform.  Same values as computed in pass_compute.
-struct TwN1024P {
+// The accessor of N = 2^a >= 256 (16 elements per lane) with the lane's
+// factors of the radix-16 pass P = 16 held in registers (kPre16): built once
+// per kernel by the row kernels, whose waves run two such transforms per
+// job, instead of four table reads and eleven products per transform.  Same
+// values as computed in pass_compute (one butterfly per lane, b = lp,
+// k = lp % 16, STEP = N / 256).
+template <int N>
+struct TwNP {
+  static_assert(N >= 256 && (N & (N - 1)) == 0, "radix-16 pass with one butterfly per lane");
   static constexpr bool kTable = false;
   static constexpr bool kPre16 = true;
   const double2* p;
   double2 w16[15];
-  __device__ __forceinline__ double2 operator()(int m) const { return TwN<1024>{p}(m); }
-  __device__ __forceinline__ TwN1024P(const double2* q, int lp) : p(q) {
-    const int k = lp & 15;   // b % P of the lane's one butterfly (P = 16, b = lp)
+  __device__ __forceinline__ double2 operator()(int m) const { return TwN<N>{p}(m); }
+  __device__ __forceinline__ TwNP(const double2* q, int lp) : p(q) {
+    const int k = lp & 15;
     double2 wp[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) wp[e] = TwN<1024>{q}((1 << e) * k * 4);   // STEP = 4
+    for (int e = 0; e < 4; ++e) wp[e] = TwN<N>{q}((1 << e) * k * (N / 256));
 #pragma unroll
     for (int r = 1; r < 16; ++r) w16[r - 1] = tw_power<16, 4>(wp, r);
   }

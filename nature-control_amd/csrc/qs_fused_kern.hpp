@@ -374,6 +374,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
   constexpr int FREE = 10240 - 6 * kBuf - kTwq - twt_len<NFIX>() - 2;   // 16-byte entries
   constexpr int NOWN = FREE / RH < 6 ? FREE / RH : 6;
   static_assert(NOWN >= 1, "own staging area");
+  constexpr bool PRE16 = EL == 16 && (NFIX == 256 || NFIX == 1024);   // (512 spills with them)
   __shared__ double2 sm[6 * kBuf + NOWN * RH];
   __shared__ double2 twq[kTwq];
   __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
@@ -424,9 +425,10 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       __syncthreads();   // job done
     }
   } else {
-    // 1024-point rows: the radix-16 pass factors in registers for the whole kernel
+    // sides 2^a with 16 elements per lane: the radix-16 pass factors in
+    // registers for the whole kernel (where they fit without spilling)
     const auto tw = [&]() {
-      if constexpr (NFIX == 1024 && EL == 16) return TwN1024P(twq, threadIdx.x & 63);
+      if constexpr (PRE16) return TwNP<NFIX>(twq, (threadIdx.x & 63) / NV);
       else return fused_tw<NFIX>(twq, twt);
     }();
     for (int w = L; w < nwork; w += gridDim.x) {

@@ -486,8 +486,7 @@ __device__ __forceinline__ void pass_load(double2 (&t)[16], const double2* buf, 
 }
 
 // A twiddle accessor with kPre16: it also holds the lane's factors of the
-// radix-16 pass P = 16 of the 1024-point transform (w16[r-1] = w^r, computed
-// once per kernel as below).
+// radix-16 pass P = 16 (w16[r-1] = w^r, computed once per kernel as below).
 template <class T, class = void>
 struct tw_has_pre16 : std::false_type {};
 template <class T>
@@ -519,7 +518,7 @@ __device__ __forceinline__ void pass_compute(double2 (&t)[16], int lp, const TW&
     // table and every other w^r is the product of the powers of its set bits
     // (at most three extra products, a few ulp): 4 table reads per butterfly
     // instead of R - 1, and only 4 twiddles live in registers.
-    if constexpr (tw_has_pre16<TW>::value && N == 1024 && P == 16 && R == 16) {
+    if constexpr (tw_has_pre16<TW>::value && P == 16 && R == 16 && S::BPL == 1) {
       // precomputed (the same products as below)
 #pragma unroll
       for (int r = 1; r < R; ++r) {
