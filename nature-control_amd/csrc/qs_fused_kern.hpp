@@ -31,24 +31,14 @@ constexpr int kRowBodyBarriers = 4;
 // the merged kernels need it (their several sizes' job code in one function
 // spills otherwise); the fixed-size kernels keep the precomputed offsets
 // (faster there).
-// NOWN: the staging layout of inverse_fields.
-template <int N, int EL = wave_e(N), bool OPQ = false, int NOWN = 0, class TW>
-__device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm, const TW& tw, double w00,
-                                         double alu, int kout) {
+// Pointwise products of a row job, in place (friction/Qsolver.cuh:63-164,
+// 206-300): points p0, p0 + pstep, ... of the job's NV rows; physical fields
+// in wave buffers 0-4, the R2C pairs out to buffers 0-5.
+template <int N, int EL>
+__device__ __forceinline__ void row_products(const RunDesc& a, int row0, double2* sm, double alu, int p0, int pstep) {
   using L = WaveLay<N, EL>;
-  constexpr int NV = L::NV;             // rows of this job (= vectors per wave FFT)
-  constexpr int LPC = L::LPC, E = L::E;
-  constexpr int VS = L::VS;             // padded vector stride inside a wave buffer
-  const int tid = OPQ ? job_tid() : (int)threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int lp = lane / NV, g = lane % NV;     // lane = lp * NV + g
-  double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
-
-  double2 v[16];
-  inverse_fields<N, EL, OPQ, NOWN>(a, row0, sm, tw, v, w00);
-
-  // ---- 4. pointwise products, in place (friction/Qsolver.cuh:63-164,206-300) --
-  for (int p = tid; p < NV * N; p += 384) {
+  constexpr int NV = L::NV, VS = L::VS;
+  for (int p = p0; p < NV * N; p += pstep) {
     const int gg = p / N, x = p % N;
     const int o = gg * VS + swz(x);
     const int yy = row0 + gg;
@@ -78,6 +68,28 @@ __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm
     sm[4 * kBuf + o] = make_double2(p21, 0.0);
     sm[5 * kBuf + o] = make_double2(loc1, loc2);
   }
+}
+
+// NOWN: the staging layout of inverse_fields.  PT: threads sharing the
+// products (384, or 512 with the loader waves, which call row_products
+// themselves between the same barriers).
+template <int N, int EL = wave_e(N), bool OPQ = false, int NOWN = 0, int PT = 384, class TW>
+__device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm, const TW& tw, double w00,
+                                         double alu, int kout) {
+  using L = WaveLay<N, EL>;
+  constexpr int NV = L::NV;             // rows of this job (= vectors per wave FFT)
+  constexpr int LPC = L::LPC, E = L::E;
+  constexpr int VS = L::VS;             // padded vector stride inside a wave buffer
+  const int tid = OPQ ? job_tid() : (int)threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int lp = lane / NV, g = lane % NV;     // lane = lp * NV + g
+  double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
+
+  double2 v[16];
+  inverse_fields<N, EL, OPQ, NOWN>(a, row0, sm, tw, v, w00);
+
+  // ---- 4. pointwise products (the loader waves take their share: PT threads)
+  row_products<N, EL>(a, row0, sm, alu, tid, PT);
   __syncthreads();
 
   // ---- 5. forward x FFTs ------------------------------------------------------
@@ -402,9 +414,13 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
     // barrier), while its transforms run; only the others wait for the job's
     // end.  The compute waves read what they need of the handoff before that
     // barrier.  The next job's loads are issued as this one starts.
+    // They also take a quarter of the products (row_body's third phase): by
+    // then they hold no load in flight.
+    int2 itc = make_int2(0, 0);
     if (L < nwork) {
       rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, NOWN);
-      if (lt == 0) rjob = work[L];
+      itc = work[L];
+      if (lt == 0) rjob = itc;
     }
     for (int w = L; w < nwork; w += gridDim.x) {
       rl.commit_part(sm, rsc, lt, NOWN, 7);   // the rest into the wave buffers
@@ -420,10 +436,18 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
         rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, NOWN);
         if (lt == 0) rjob = itn;
       }
-#pragma unroll 1
-      for (int b = 1; b < kRowBodyBarriers; ++b) __syncthreads();
+      __syncthreads();   // inverse transforms done (row_body's second barrier)
+      {
+        const RunDesc& a = runs[itc.x];
+        const double alu = a.alpha_p ? gload(a.alpha_p) : a.alpha;
+        row_products<NFIX, EL>(a, itc.y * NV, sm, alu, threadIdx.x, kRowThreads);
+      }
+      __syncthreads();   // products done
+      __syncthreads();   // forward transforms done
       __syncthreads();   // job done
+      itc = itn;
     }
+    static_assert(kRowBodyBarriers == 4, "the loader waves pass row_body's four barriers");
   } else {
     // sides 2^a with 16 elements per lane: the radix-16 pass factors in
     // registers for the whole kernel (where they fit without spilling)
@@ -435,7 +459,8 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       __syncthreads();   // staging ready
       const int2 it = rjob;   // (run, job)
       const RunDesc& a = runs[it.x];
-      row_body<NFIX, EL, false, NOWN>(a, it.y * NV, sm, tw, rsc[0], rsc[1], fin ? col_kept_cols(a) : NFIX / 2 + 1);
+      row_body<NFIX, EL, false, NOWN, kRowThreads>(a, it.y * NV, sm, tw, rsc[0], rsc[1],
+                                                   fin ? col_kept_cols(a) : NFIX / 2 + 1);
       __syncthreads();   // job done: wave buffers free for the next staging
     }
   }
