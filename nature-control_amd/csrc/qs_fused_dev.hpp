@@ -266,7 +266,11 @@ struct RowLoader {
 // barrier.
 // NOWN: staged arrays t < NOWN at sm + 6 kBuf + t RH (an area of their own
 // after the six wave buffers), the others at sm + t RH (in the buffers).
-template <int N, int EL = wave_e(N), bool OPQ = false, int NOWN = 0, class TW>
+// EARLY: the others in wave buffer 5 instead (at sm + 5 kBuf + (t - NOWN) RH;
+// buffer 5 is not written before the products), so no staged array overlaps
+// buffers 0-4 and each wave transforms as soon as it has built its inputs:
+// one barrier (after the transforms) instead of two.
+template <int N, int EL = wave_e(N), bool OPQ = false, int NOWN = 0, bool EARLY = false, class TW>
 __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, double2* sm, const TW& tw,
                                                double2 (&v)[16], double w00) {
   using L = WaveLay<N, EL>;
@@ -281,8 +285,10 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
   // instruction's offset field, its constant addresses would be kept, and
   // spilled, across the job loop)
   const double2* stg = sm + (NOWN > 0 ? opq(6 * kBuf) : 0);
+  const double2* stg5 = sm + (EARLY ? opq(5 * kBuf) : 0);
+  static_assert(!EARLY || (7 - NOWN) * RH <= kBuf, "the other arrays fit wave buffer 5");
   auto arr = [&](int t) -> const double2* {   // staged array t, vector g
-    return (t < NOWN ? stg : sm) + t * RH + g * H;
+    return (t < NOWN ? stg + t * RH : (EARLY ? stg5 + (t - NOWN) * RH : sm + t * RH)) + g * H;
   };
   auto build = [&](auto&& pair) {
 #pragma unroll
@@ -326,7 +332,7 @@ __device__ __forceinline__ void inverse_fields(const RunDesc& a, int row0, doubl
       Y = make_double2(0.0, 0.0);
     });
   }
-  __syncthreads();   // staging area free
+  if (!EARLY) __syncthreads();   // staging area free
   if (wave < 5) {
     const double sc = a.scale;
     wave_fft<N, +1, EL>(v, vb, lp, tw);

@@ -24,8 +24,8 @@ namespace {
 //  6. cooperative unpack + spectral combination -> F0..F6 (coalesced stores).
 // The part of a row job after staging: phases 3-6, run by threads 0..383.
 // w00 = *a.w00; alu: the step's uniform activity (*a.alpha_p, or a.alpha).
-// Block barriers: kRowBodyBarriers (two in inverse_fields, one after the
-// products, one after the forward FFTs).
+// Block barriers: kRowBodyBarriers (two in inverse_fields, one with EARLY;
+// one after the products, one after the forward FFTs).
 constexpr int kRowBodyBarriers = 4;
 // OPQ: lane offsets from an opaque thread index (job_tid), recomputed per job:
 // the merged kernels need it (their several sizes' job code in one function
@@ -70,10 +70,10 @@ __device__ __forceinline__ void row_products(const RunDesc& a, int row0, double2
   }
 }
 
-// NOWN: the staging layout of inverse_fields.  PT: threads sharing the
-// products (384, or 512 with the loader waves, which call row_products
-// themselves between the same barriers).
-template <int N, int EL = wave_e(N), bool OPQ = false, int NOWN = 0, int PT = 384, class TW>
+// NOWN, EARLY: the staging layout of inverse_fields (EARLY: one barrier
+// fewer).  PT: threads sharing the products (384, or 512 with the loader
+// waves, which call row_products themselves between the same barriers).
+template <int N, int EL = wave_e(N), bool OPQ = false, int NOWN = 0, int PT = 384, bool EARLY = false, class TW>
 __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm, const TW& tw, double w00,
                                          double alu, int kout) {
   using L = WaveLay<N, EL>;
@@ -86,7 +86,7 @@ __device__ __forceinline__ void row_body(const RunDesc& a, int row0, double2* sm
   double2* vb = sm + wave * kBuf + g * VS;     // this lane's vector in its wave buffer
 
   double2 v[16];
-  inverse_fields<N, EL, OPQ, NOWN>(a, row0, sm, tw, v, w00);
+  inverse_fields<N, EL, OPQ, NOWN, EARLY>(a, row0, sm, tw, v, w00);
 
   // ---- 4. pointwise products (the loader waves take their share: PT threads)
   row_products<N, EL>(a, row0, sm, alu, tid, PT);
@@ -386,8 +386,11 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
   constexpr int FREE = 10240 - 6 * kBuf - kTwq - twt_len<NFIX>() - 2;   // 16-byte entries
   constexpr int NOWN = FREE / RH < 6 ? FREE / RH : 6;
   static_assert(NOWN >= 1, "own staging area");
+  // the other arrays in wave buffer 5 where they fit (all sides but 16)
+  constexpr bool EARLY = (7 - NOWN) * RH <= kBuf;
   constexpr bool PRE16 = EL == 16 && (NFIX == 256 || NFIX == 1024);   // (512 spills with them)
   __shared__ double2 sm[6 * kBuf + NOWN * RH];
+  double2* const late = EARLY ? sm + 5 * kBuf - NOWN * RH : sm;   // array t >= NOWN at late + t RH
   __shared__ double2 twq[kTwq];
   __shared__ double2 twt[twt_len<NFIX>()];   // per-pass twiddles (use_twt)
   __shared__ double rsc[2];                  // the job's (w00, alu), from the loader
@@ -423,7 +426,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       if (lt == 0) rjob = itc;
     }
     for (int w = L; w < nwork; w += gridDim.x) {
-      rl.commit_part(sm, rsc, lt, NOWN, 7);   // the rest into the wave buffers
+      rl.commit_part(late, rsc, lt, NOWN, 7);   // the rest into the wave buffers
       __syncthreads();   // staging ready
       const int wn = w + gridDim.x;
       int2 itn = make_int2(0, 0);
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
         rl.commit_part(sm + 6 * kBuf, rsc, lt, 0, NOWN);
         if (lt == 0) rjob = itn;
       }
-      __syncthreads();   // inverse transforms done (row_body's second barrier)
+      if (!EARLY) __syncthreads();   // inverse transforms done (row_body's second barrier)
       {
         const RunDesc& a = runs[itc.x];
         const double alu = a.alpha_p ? gload(a.alpha_p) : a.alpha;
@@ -447,7 +450,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       __syncthreads();   // job done
       itc = itn;
     }
-    static_assert(kRowBodyBarriers == 4, "the loader waves pass row_body's four barriers");
+    static_assert(kRowBodyBarriers == 4, "the loader waves pass row_body's four barriers (three when EARLY)");
   } else {
     // sides 2^a with 16 elements per lane: the radix-16 pass factors in
     // registers for the whole kernel (where they fit without spilling)
@@ -459,8 +462,8 @@ __global__ __launch_bounds__(kRowThreads) void k_row(const RunDesc* __restrict__
       __syncthreads();   // staging ready
       const int2 it = rjob;   // (run, job)
       const RunDesc& a = runs[it.x];
-      row_body<NFIX, EL, false, NOWN, kRowThreads>(a, it.y * NV, sm, tw, rsc[0], rsc[1],
-                                                   fin ? col_kept_cols(a) : NFIX / 2 + 1);
+      row_body<NFIX, EL, false, NOWN, kRowThreads, EARLY>(a, it.y * NV, sm, tw, rsc[0], rsc[1],
+                                                          fin ? col_kept_cols(a) : NFIX / 2 + 1);
       __syncthreads();   // job done: wave buffers free for the next staging
     }
   }
