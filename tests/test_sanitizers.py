@@ -1,8 +1,10 @@
 """Host AddressSanitizer + UndefinedBehaviorSanitizer runs (CPU): the oracle
-under tests/sanitize/oracle_driver.cpp and libqsolver's host-only code (error
+under tests/sanitize/oracle_driver.cpp, libqsolver's host-only code (error
 handling, Mesh set-up and teardown, CSV text I/O) under
-tests/sanitize/host_driver.cpp, both built by tests/sanitize/Makefile with the
-sanitizers aborting on their first finding (leak detection on)."""
+tests/sanitize/host_driver.cpp, and the column-job placement of the fused
+column kernels under tests/sanitize/place_driver.cpp, all built by
+tests/sanitize/Makefile with the sanitizers aborting on their first finding
+(leak detection on)."""
 import os
 import shutil
 import subprocess
@@ -36,3 +38,14 @@ def test_host_code_under_sanitizers(built, tmp_path):
     r = run(os.path.join(built, "host_san"), str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
     assert "host sanitizer run: ok" in r.stdout
+
+
+def test_column_job_placement(built):
+    """col_job_place maps the jobs of every fused side, tile width and stage
+    kind one to one onto (field, tile), with the 1024^2 order of an XCD range
+    (r1, vorticity, r2, the partial tile; final stage: vorticity jobs ending
+    the first round)."""
+    r = run(os.path.join(built, "place_san"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "placement run: ok" in r.stdout
+
